@@ -1,0 +1,194 @@
+"""Throughput of the IISPH step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--preset dambreak_1m]
+
+Workload: the 1,000,000-particle dam break (100^3 fluid lattice at 0.01 m,
++-1% jitter from seed 0, 2x1x1 m box with one wall layer, dt = 1e-3,
+XSPH 0.01, relaxed Jacobi to a mean density error <= 1e-3). W untimed steps
+run first; then exactly K steps are timed between two device
+synchronisations with the particle state resident in HBM. ``value`` counts
+fluid particle-steps per second over the steps whose pressure solve
+converged (residual <= eta).
+
+The roofline figure covers the dominant kernel, the persistent Jacobi loop:
+its algorithmic bytes are 92 B per fluid particle per A p evaluation
+(J1: x, p, rho, d_ii read 32 B, s and A p partial written 16 B; J2: x, s,
+A p, a_ii, p, rho_adv read 40 B, p_next written 4 B), and its duration is
+measured with HIP events on the solver's stream over the timed steps.
+
+The CPU baseline times the float32 CPU oracle (one thread) on a bounded
+sample: the GPU's particle state at the end of the timed run, advanced a
+few steps.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_AP_EVAL = 92  # per fluid particle, J1 48 + J2 44
+BYTES_PER_STEP_FIXED = 264  # grid 84 + density 16 + v_adv 40 + assembly 48 + integrate 76
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--preset", default="dambreak_1m")
+    ap.add_argument("--jacobi", choices=["persistent", "host"], default="persistent")
+    ap.add_argument("--warm-start", choices=["zero", "previous"], default="zero")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    return ap.parse_args()
+
+
+def traffic_from_profile(workload: str):
+    """HBM bytes per launch of the Jacobi kernel from the committed rocprofv3
+    counter summary, if one exists for this workload."""
+    f = ROOT / "profiles" / f"jacobi_persistent_{workload}_traffic.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(cfg, x, v, budget_s: float):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import oracle as O
+    from warpsph_amd import scene as scn
+    O.set_threads(1)
+    sc = scn.build(cfg)
+    sc.x = x
+    O_s = O.OracleSolver(cfg, sc)
+    O_s.v = v.copy()
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        O_s.step(want_fields=False)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s * 0.5 or el * (steps + 1) / steps > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return {"value": cfg_n(x) * steps / el, "unit": "fluid particle-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{cfg_n(x)}-particle state at the end of the GPU run, {steps} step(s), "
+                      f"{el:.1f} s, one thread, float32 oracle restating the reference step"}
+
+
+def cfg_n(x):
+    return int(x.shape[0])
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+    from warpsph_amd import config
+    from warpsph_amd import _native as N
+    from warpsph_amd.solver import IISPH
+
+    cfg = config.preset(args.preset)
+    cfg = cfg.replace(jacobi_mode=config.JACOBI_PERSISTENT if args.jacobi == "persistent"
+                      else config.JACOBI_HOST,
+                      warm_start=config.WARM_ZERO if args.warm_start == "zero"
+                      else config.WARM_PREVIOUS)
+    t_setup = time.perf_counter()
+    s = IISPH(stage_path=None, config=cfg)
+    lib, h = s._lib, s._h
+    print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s n={s.n} nb={s.boundary_n}",
+          file=sys.stderr, flush=True)
+
+    for k in range(args.warmup):
+        N.check("wsph_step", lib.wsph_step(h, None))
+    N.check("wsph_sync", lib.wsph_sync(h))
+    c0 = s.counters()
+    N.check("wsph_set_timing", lib.wsph_set_timing(h, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        N.check("wsph_step", lib.wsph_step(h, None))
+    N.check("wsph_sync", lib.wsph_sync(h))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    c1 = s.counters()
+    tm = (C.c_double * 8)()
+    N.check("wsph_last_timings", lib.wsph_last_timings(h, tm, 8))
+    if c1["status"]:
+        raise RuntimeError(f"device status {c1['status']} during the timed run")
+    elapsed = t1 - t0
+    steps = c1["steps"] - c0["steps"]
+    iters = c1["iterations"] - c0["iterations"]
+    nonconv = c1["nonconverged"] - c0["nonconverged"]
+    ap_evals = c1["ap_evals"] - c0["ap_evals"]
+    n = s.n
+    value = n * (steps - nonconv) / elapsed
+    jac_ms = tm[2] / max(1, int(tm[6]))
+    jac_bytes = n * BYTES_PER_AP_EVAL * ap_evals / max(1, steps)
+    achieved = jac_bytes / (jac_ms * 1e-3) / 1e9
+    workload = args.preset
+    line = {
+        "metric": "fluid particle-steps/s @1M, rho-err<=0.1%",
+        "value": value,
+        "unit": "fluid particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": workload, "fluid_particles": n, "boundary_particles": s.boundary_n,
+                   "spacing": cfg.spacing, "dt": cfg.dt0, "jacobi": args.jacobi,
+                   "warm_start": args.warm_start, "eta": cfg.eta},
+        "jacobi_iters_per_step": iters / max(1, steps),
+        "nonconverged_steps": nonconv,
+        "phase_ms_per_step": {"neighbor_search": tm[0] / max(1, tm[6]),
+                              "predict_advection": tm[1] / max(1, tm[6]),
+                              "pressure_solve": jac_ms,
+                              "integrate": tm[3] / max(1, tm[6])},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(workload),
+                     "kernel": "k_jacobi_persistent",
+                     "algorithmic_bytes_per_launch": jac_bytes,
+                     "avg_launch_ms": jac_ms},
+        "step_roofline_frac": (n * (BYTES_PER_STEP_FIXED * steps + BYTES_PER_AP_EVAL * ap_evals)
+                               / elapsed / 1e9) / HBM_PEAK_GBS,
+    }
+    if not args.no_cpu_baseline:
+        x = s.x.cpu().numpy()
+        v = s.v.cpu().numpy()
+        line["cpu_baseline"] = cpu_baseline(cfg, x, v, args.cpu_seconds)
+        line["cpu_baseline"]["host_cpu"] = _cpu_model()
+    print(json.dumps(line), flush=True)
+
+
+def _cpu_model():
+    try:
+        for l in open("/proc/cpuinfo"):
+            if l.startswith("model name"):
+                return l.split(":", 1)[1].strip() + f" x{os.cpu_count()}"
+    except OSError:
+        pass
+    return None
+
+
+if __name__ == "__main__":
+    main()

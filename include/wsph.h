@@ -1,0 +1,198 @@
+/*
+ * wsph.h — C-ABI of libwsph.so, the MI355X (gfx950) IISPH step.
+ *
+ * Every entry point takes plain pointers and sizes; array arguments marked
+ * "device" are HIP device pointers (a torch ROCm tensor's data_ptr()), arrays
+ * marked "host" are ordinary host memory. Every call returns an int status
+ * (WSPH_OK = 0, negative on error) and leaves a message readable with
+ * wsph_last_error() (thread-local), the way the reference raises
+ * RuntimeError from IISPH.raise_error (wp_sph/solver/IISPH.py:1484-1487).
+ *
+ * The reference has no FFI: its solver is the Python class
+ * wp_sph.solver.IISPH (IISPH.py:736-1501) whose methods each issue
+ * wp.launch(kernel, dim, inputs, outputs) calls. Each entry point below
+ * names the reference code it replaces.
+ */
+#ifndef WSPH_H
+#define WSPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WSPH_ABI_VERSION 1
+
+/* The mean density error (update_rho_error, IISPH.py:542-568) is summed in
+ * 64-bit fixed point so that it is independent of summation order: each
+ * term |rho_star / rho0 - 1| (float32) is clamped at WSPH_RESIDUAL_TERM_CLAMP and
+ * truncated to a multiple of 1/WSPH_RESIDUAL_FIXED_SCALE. */
+#define WSPH_RESIDUAL_FIXED_SCALE 268435456.0 /* 2^28 */
+#define WSPH_RESIDUAL_TERM_CLAMP 1024.0f
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+  WSPH_OK = 0,
+  WSPH_E_ARG = -1,        /* invalid argument or call order */
+  WSPH_E_HIP = -2,        /* a HIP runtime call failed */
+  WSPH_E_CAPACITY = -3,   /* neighbour list or grid capacity exceeded */
+  WSPH_E_NAN = -4,        /* non-finite position or velocity after a step */
+  WSPH_E_BARRIER = -5,    /* persistent kernel grid barrier timed out */
+  WSPH_E_UNSUPPORTED = -6 /* device cannot run the requested mode */
+};
+
+/* ---- enumerations ------------------------------------------------------ */
+enum { WSPH_LAYOUT_LINEAR = 0, WSPH_LAYOUT_MORTON = 1 };
+/* pressure start value each step: 0 as IISPH.py:166-174, the previous
+ * step's pressure, or a caller-supplied field (learned predictor) */
+enum { WSPH_WARM_ZERO = 0, WSPH_WARM_PREVIOUS = 1, WSPH_WARM_EXTERNAL = 2 };
+/* dt: CFL rule of IISPH.py:1368-1370, or fixed */
+enum { WSPH_DT_CFL = 0, WSPH_DT_FIXED = 1 };
+/* Jacobi driver: one launch per kernel with a host convergence test per
+ * iteration (the reference's loop, IISPH.py:1200-1296), or one persistent
+ * cooperative kernel with a device-side residual */
+enum { WSPH_JACOBI_HOST = 0, WSPH_JACOBI_PERSISTENT = 1 };
+
+/* fields readable with wsph_get_field (original particle order) */
+enum {
+  WSPH_F_X = 0,       /* float[n][3] positions              IISPH.py:993 */
+  WSPH_F_V = 1,       /* float[n][3] velocities             IISPH.py:783 */
+  WSPH_F_P = 2,       /* float[n]    pressure               IISPH.py:788 */
+  WSPH_F_RHO = 3,     /* float[n]    density                IISPH.py:785 */
+  WSPH_F_RHO_ADV = 4, /* float[n]    advected density       IISPH.py:786 */
+  WSPH_F_V_ADV = 5,   /* float[n][3] advected velocity      IISPH.py:784 */
+  WSPH_F_D_II = 6,    /* float[n][3] d_ii                   IISPH.py:793 */
+  WSPH_F_A_II = 7,    /* float[n]    a_ii                   IISPH.py:792 */
+  WSPH_F_AP = 8,      /* float[n]    A p                    IISPH.py:796 */
+  WSPH_F_S = 9,       /* float[n][3] sum_j d_ij p_j         IISPH.py:797 */
+  WSPH_F_FF_COUNT = 10, /* int[n]    fluid neighbour count  IISPH.py:798 */
+  WSPH_F_FS_COUNT = 11, /* int[n]    boundary neighbour count IISPH.py:802 */
+  WSPH_F_PSI = 12,    /* float[nb]   boundary psi           IISPH.py:789 */
+  WSPH_F_CELL_KEY = 13, /* int[n]    linear hash key of each particle */
+  WSPH_F_COUNT
+};
+
+/* Solver parameters. The physics fields restate wp_sph/main.py:25-61; the
+ * host converts each double to float once, as the reference's kernel
+ * constants are float32. */
+typedef struct wsph_params {
+  double spacing;   /* DIAMETER                        main.py:26 */
+  double h;         /* SMOOTHING_LENGTH = 2 DIAMETER   main.py:57 */
+  double rho0;      /* RHO_0                           main.py:29 */
+  double mass;      /* FLUID_MASS = RHO_0 d^3          main.py:59 */
+  double gravity;   /* GRAVITY, acting along -y        main.py:25, IISPH.py:277 */
+  double omega;     /* OMEGA (relaxed Jacobi)          main.py:27 */
+  double eta;       /* ETA (mean density error bound)  main.py:28 */
+  double inv_small; /* INV_SMALL guard on a_ii         main.py:30 */
+  double vis_mu;    /* VIS_MU Laplacian viscosity      main.py:61, IISPH.py:273-278 */
+  double xsph;      /* XSPH velocity smoothing coefficient (0 = off) */
+  double dt;        /* fixed dt, or the first dt under CFL (IISPH.py:747) */
+  double dt_max;    /* TIME_STEP_MAX = 0.05 d          main.py:58 */
+  double cfl;       /* 0.4, IISPH.py:1369 */
+  int dt_mode;      /* WSPH_DT_* */
+  int min_iters;    /* 2   (IISPH.py:1201) */
+  int max_iters;    /* 101 (IISPH.py:1202) */
+  int force_iters;  /* >0: run exactly this many Jacobi iterations */
+  int warm_start;   /* WSPH_WARM_* */
+  int grid_dims[3]; /* fluid hash grid dims (IISPH.py:771-775) */
+  int bgrid_dims[3];/* boundary hash grid dims (IISPH.py:776-780) */
+  int cell_layout;  /* WSPH_LAYOUT_* : order of cells in memory */
+  int max_ff;       /* fluid-fluid neighbour capacity per particle */
+  int max_fs;       /* fluid-boundary neighbour capacity per particle */
+  int jacobi_mode;  /* WSPH_JACOBI_* */
+  int block;        /* threads per block of the neighbour passes (64..1024) */
+  int coop_blocks_per_cu; /* persistent kernel blocks per CU (0 = occupancy max) */
+} wsph_params;
+
+/* Per-step statistics (what IISPH.step computes and prints, IISPH.py:1200-1371) */
+typedef struct wsph_step_stats {
+  int iterations;   /* Jacobi iterations run this step (2..101) */
+  int converged;    /* final residual <= eta */
+  int ap_evals;     /* A p evaluations (iterations, +1 for a non-zero start) */
+  int max_ff_count; /* largest fluid-fluid neighbour count */
+  int max_fs_count; /* largest fluid-boundary neighbour count */
+  int status;       /* device-reported status of this step */
+  double residual;  /* mean density error at exit (IISPH.py:1489-1494) */
+  double dt;        /* dt used by this step */
+  double sim_time;  /* simulated time after this step */
+  double v_max;     /* max |v| after the kick (IISPH.py:688) */
+} wsph_step_stats;
+
+typedef struct wsph_solver wsph_solver;
+
+const char *wsph_last_error(void);
+int wsph_abi_version(void);
+
+/* Create a solver (replaces IISPH.__init__'s allocations, IISPH.py:770-805). */
+int wsph_create(const wsph_params *params, wsph_solver **out);
+/* Release every device buffer the solver allocated. */
+int wsph_destroy(wsph_solver *s);
+
+/* Upload the W and grad-W/r tables (host float32), replaces the module-level
+ * W_table / grad_W_table of sph_funcs.py:51-89. */
+int wsph_set_tables(wsph_solver *s, const float *w_table, int w_size,
+                    const float *grad_table, int grad_size);
+
+/* Upload boundary particles (device float[nb][3] each; vb may be NULL for
+ * zero velocity), build the boundary hash grid and compute psi
+ * (IISPH.py:811-821 -> compute_boundary_density, IISPH.py:23-47). */
+int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb);
+
+/* Upload fluid state (device float[n][3]; v may be NULL for rest), replaces
+ * the wp.array copies at IISPH.py:990-994. */
+int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n);
+
+/* Set the pressure start value for the next step (device float[n], original
+ * order). Used with WSPH_WARM_EXTERNAL. */
+int wsph_set_pressure(wsph_solver *s, const float *p);
+
+/* One full time step: IISPH.step (IISPH.py:1002-1371). */
+int wsph_step(wsph_solver *s, wsph_step_stats *stats);
+
+/* The four phases of IISPH.step as the reference times them
+ * (IISPH.py:1004, 1007, 1154, 1298). wsph_step runs all four. */
+int wsph_neighbor_search(wsph_solver *s);        /* IISPH.py:1408-1482 */
+int wsph_predict_advection(wsph_solver *s);      /* IISPH.py:1009-1152 */
+int wsph_pressure_solve(wsph_solver *s, wsph_step_stats *stats); /* IISPH.py:1155-1296 */
+int wsph_integrate(wsph_solver *s, wsph_step_stats *stats);      /* IISPH.py:1299-1371 */
+
+/* Copy a field into dst (device), scattered back to original particle order. */
+int wsph_get_field(wsph_solver *s, int field, void *dst);
+
+/* Neighbour lists in original indices and the reference's enumeration
+ * order (count_/store_same_neighbor IISPH.py:52-104 when which = 0,
+ * count_/store_diff_neighbor IISPH.py:107-163 when which = 1).
+ * cnt: device int[n]; idx: device int[n][cap], row i holding particle i's
+ * neighbours in order. */
+int wsph_get_neighbors(wsph_solver *s, int which, int *cnt, int *idx, int cap);
+
+/* Simulation clock (sim_time, dt) kept on the device (IISPH.py:1367-1371). */
+int wsph_get_time(wsph_solver *s, double *sim_time, double *dt);
+
+/* Stand-alone hash-key kernel: linear key cz*dx*dy + cy*dx + cx of the
+ * uniform hash grid (warp.HashGrid, used at IISPH.py:771-780, 1414) for n
+ * device points, written to device int keys[n]. */
+int wsph_hash_keys(const float *x, int n, float cell_width, const int dims[3],
+                   int *keys);
+
+/* Phase timing with HIP events on the solver's stream, summed over every
+ * step since wsph_set_timing(s, 1) (which resets the sums): out[0..3] ms of
+ * neighbour search, advection, pressure solve and integration, out[4] ms of
+ * whole steps, out[5] persistent-kernel grid size, out[6] steps recorded. */
+int wsph_last_timings(wsph_solver *s, double *out, int nout);
+int wsph_set_timing(wsph_solver *s, int enable);
+
+/* Cumulative counters since wsph_create, kept on the device so that a run
+ * of steps needs no host synchronisation: out[0] steps, out[1] Jacobi
+ * iterations, out[2] steps that ended above eta, out[3] A p evaluations,
+ * out[4] OR of the device status bits. */
+int wsph_get_counters(wsph_solver *s, long long *out, int nout);
+
+/* Block until the solver's stream is idle. */
+int wsph_sync(wsph_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WSPH_H */

@@ -1,0 +1,48 @@
+"""libwsph.so loads and exports every entry point include/wsph.h declares.
+(No compute call: this runs without a GPU.)"""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from warpsph_amd import _native as N
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "wsph.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wsph_\w+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    bound = sorted(name for name, _, _ in N.SIGNATURES)
+    assert declared_symbols() == bound
+
+
+def test_library_exports_every_symbol():
+    if not N.LIB_PATH.exists():
+        pytest.fail("libwsph.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(N.LIB_PATH))
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_abi_version_and_error_string():
+    lib = N.load()
+    assert lib.wsph_abi_version() == 1
+    assert isinstance(lib.wsph_last_error(), bytes)
+
+
+def test_create_rejects_bad_params_without_gpu_work():
+    lib = N.load()
+    from warpsph_amd import config
+    p = N.make_params(config.dambreak_8k(), (6, 6, 6), (6, 6, 6))
+    p.max_iters = 1000
+    h = ctypes.c_void_p()
+    assert lib.wsph_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert b"iteration" in lib.wsph_last_error()
+    p = N.make_params(config.dambreak_8k(), (0, 6, 6), (6, 6, 6))
+    assert lib.wsph_create(ctypes.byref(p), ctypes.byref(h)) == -1

@@ -1,0 +1,172 @@
+"""Parity of the HIP path (libwsph.so through the C-ABI) with the CPU oracle.
+
+Both sides run the same scene from the same float32 inputs. Because the GPU
+kernels enumerate neighbours in the reference's order and keep its float32
+operation order with no fused multiply-add, every field is compared for
+exact equality (bit-exact), not within a tolerance.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import assert_same, oracle_setup
+from warpsph_amd import config, scene
+from warpsph_amd import _native as N
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def gpu_solver(cfg, sc=None):
+    from warpsph_amd.solver import IISPH
+    return IISPH(stage_path=None, config=cfg, scene=sc)
+
+
+def fields_of(s):
+    return dict(rho=s.rho, v_adv=s.v_adv, rho_adv=s.rho_adv, d_ii=s.term_d_ii,
+                a_ii=s.term_a_ii, p=s.p, Ap=s.term_Ap_i, s=s.sum_d_ij_p_j,
+                ff_count=s.ff_neighbor_num, fs_count=s.fs_neighbor_num)
+
+
+def compare_step(s, info, x_o, v_o, skip=()):
+    f = {k: v.cpu().numpy() for k, v in fields_of(s).items()}
+    for k in ("rho", "v_adv", "rho_adv", "d_ii", "a_ii", "p", "Ap", "s", "ff_count", "fs_count"):
+        if k not in skip:
+            assert_same(k, f[k], info[k])
+    assert_same("x", s.x.cpu().numpy(), x_o)
+    assert_same("v", s.v.cpu().numpy(), v_o)
+    st = s.step_stats()
+    assert st["iterations"] == info["iterations"]
+    assert st["residual"] == info["residual"]
+
+
+def test_boundary_psi_bitexact(orc):
+    cfg = config.dambreak_8k()
+    _announce("psi dambreak_8k")
+    s = gpu_solver(cfg)
+    sc, P, wt, gt = oracle_setup(cfg)
+    assert_same("psi", s.boundary_phi.cpu().numpy(), orc.boundary_psi(P, wt, sc.xb))
+
+
+@pytest.mark.parametrize("layout", [config.LAYOUT_MORTON, config.LAYOUT_LINEAR])
+def test_neighbor_lists_bitexact(orc, layout):
+    cfg = config.dambreak_8k().replace(cell_layout=layout, jitter=0.01)
+    _announce(f"neighbour lists layout={layout}")
+    s = gpu_solver(cfg)
+    s.neighbor_search()
+    sc, P, wt, gt = oracle_setup(cfg)
+    for which, (xs, excl, dims) in enumerate([(sc.x, True, P.grid_dims),
+                                              (sc.xb, False, P.bgrid_dims)]):
+        cnt_o, idx_o, _ = orc.neighbors(sc.x, xs, cfg.h, tuple(dims), excl, cap=64)
+        cnt, idx = s.neighbors(which, cap=64)
+        cnt, idx = cnt.cpu().numpy(), idx.cpu().numpy()
+        assert_same(f"count[{which}]", cnt, cnt_o)
+        for i in range(sc.n):
+            assert idx[i, :cnt[i]].tolist() == idx_o[i, :cnt_o[i]].tolist(), (which, i)
+
+
+def test_cell_keys_bitexact(orc):
+    cfg = config.dambreak_8k().replace(grid="reference", jitter=0.02)
+    _announce("cell keys")
+    s = gpu_solver(cfg)
+    keys = s.field(N.F_CELL_KEY).cpu().numpy()
+    sc = scene.build(cfg)
+    assert_same("keys", keys, orc.hash_keys(sc.x, cfg.h, cfg.reference_grid_dims()))
+
+
+@pytest.mark.parametrize("mode", [config.JACOBI_PERSISTENT, config.JACOBI_HOST])
+@pytest.mark.parametrize("force", [3, 0])
+def test_one_step_bitexact(orc, mode, force):
+    cfg = config.dambreak_8k().replace(jacobi_mode=mode, force_iters=force)
+    _announce(f"one step mode={mode} force={force}")
+    s = gpu_solver(cfg)
+    s.step()
+    sc, P, wt, gt = oracle_setup(cfg)
+    psi = orc.boundary_psi(P, wt, sc.xb)
+    x_o, v_o, info = orc.step(P, wt, gt, sc.x, np.zeros_like(sc.x), sc.xb, sc.vb, psi)
+    compare_step(s, info, x_o, v_o)
+
+
+@pytest.mark.parametrize("preset,steps", [("dambreak_8k", 25)])
+def test_trajectory_bitexact(orc, preset, steps):
+    cfg = config.preset(preset)
+    _announce(f"trajectory {preset} {steps} steps")
+    s = gpu_solver(cfg)
+    sc = scene.build(cfg)
+    O = orc.OracleSolver(cfg, sc)
+    for k in range(steps):
+        s.step()
+        info = O.step()
+        assert s.step_stats()["iterations"] == info["iterations"], k
+    assert_same("x", s.x.cpu().numpy(), O.x)
+    assert_same("v", s.v.cpu().numpy(), O.v)
+    assert_same("p", s.p.cpu().numpy(), O.p)
+
+
+def test_warm_start_previous_bitexact(orc):
+    cfg = config.dambreak_8k().replace(warm_start=config.WARM_PREVIOUS)
+    _announce("warm start previous")
+    s = gpu_solver(cfg)
+    sc = scene.build(cfg)
+    O = orc.OracleSolver(cfg, sc)
+    for k in range(4):
+        s.step()
+        info = O.step()
+        assert s.step_stats()["iterations"] == info["iterations"], k
+        assert s.step_stats()["ap_evals"] == info["ap_evals"], k
+    compare_step(s, info, O.x, O.v)
+
+
+def test_external_start_bitexact(orc):
+    cfg = config.dambreak_8k().replace(warm_start=config.WARM_EXTERNAL)
+    _announce("external start")
+    s = gpu_solver(cfg)
+    sc, P, wt, gt = oracle_setup(cfg)
+    psi = orc.boundary_psi(P, wt, sc.xb)
+    p0 = np.random.default_rng(5).uniform(0, 2000, sc.n).astype(np.float32)
+    s.set_pressure(p0)
+    s.step()
+    x_o, v_o, info = orc.step(P, wt, gt, sc.x, np.zeros_like(sc.x), sc.xb, sc.vb, psi, p0=p0)
+    compare_step(s, info, x_o, v_o)
+
+
+def test_reference_scene_one_step(orc):
+    """The reference's default scene: 102,641 fluid and 124,056 wall
+    particles, CFL dt, 8192/65536-sample tables (global-memory table path)."""
+    cfg = config.reference().replace(force_iters=2)
+    _announce("reference scene")
+    s = gpu_solver(cfg)
+    assert (s.n, s.boundary_n) == (102641, 124056)
+    s.step()
+    sc, P, wt, gt = oracle_setup(cfg)
+    psi = orc.boundary_psi(P, wt, sc.xb)
+    x_o, v_o, info = orc.step(P, wt, gt, sc.x, np.zeros_like(sc.x), sc.xb, sc.vb, psi)
+    compare_step(s, info, x_o, v_o)
+    t, dt = s.get_time()
+    assert t == cfg.dt_max
+    assert dt == min(cfg.cfl * cfg.spacing / max(float(info["v_max"]), cfg.inv_small), cfg.dt_max)
+
+
+def test_viscosity_term_bitexact(orc):
+    """The Laplacian viscosity term as written (IISPH.py:249-278) with VIS_MU != 0."""
+    cfg = config.dambreak_8k().replace(vis_mu=0.5, xsph=0.0, force_iters=2)
+    _announce("viscosity")
+    sc = scene.build(cfg)
+    v0 = np.random.default_rng(9).normal(0, 0.1, sc.x.shape).astype(np.float32)
+    s = gpu_solver(cfg, sc)
+    import ctypes
+    import torch
+    vt = torch.from_numpy(v0).cuda()
+    xt = torch.from_numpy(sc.x).cuda()
+    torch.cuda.synchronize()
+    N.check("wsph_set_fluid", s._lib.wsph_set_fluid(s._h, xt.data_ptr(), vt.data_ptr(), sc.n))
+    s.step()
+    _, P, wt, gt = oracle_setup(cfg, sc)
+    psi = orc.boundary_psi(P, wt, sc.xb)
+    x_o, v_o, info = orc.step(P, wt, gt, sc.x, v0, sc.xb, sc.vb, psi)
+    compare_step(s, info, x_o, v_o)

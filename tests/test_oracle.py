@@ -1,0 +1,278 @@
+"""Pins of the CPU oracle (oracle/iisph_oracle.c) against what the mathematics
+fixes: closed forms of the cubic-spline lattice sums, brute-force neighbour
+sets, the reference's own self-consistency identity (check_rho_error,
+IISPH.py:570-648) and momentum conservation. The reference ships no tests or
+golden vectors, so these are the pins."""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import brute_force_lists, oracle_setup
+from warpsph_amd import config, scene, tables
+
+
+def sigma(h):
+    return 8.0 / (math.pi * h ** 3)
+
+
+def w_exact(r, h):
+    q = r / h
+    s = sigma(h)
+    if q <= 0.5:
+        return s * (6 * (q ** 3 - q ** 2) + 1)
+    if q < 1:
+        return 2 * s * (1 - q) ** 3
+    return 0.0
+
+
+def dw_exact(r, h):
+    """|dW/dr| of the cubic spline."""
+    q = r / h
+    s = sigma(h) / h
+    if q <= 0.5:
+        return abs(s * (18 * q * q - 12 * q))
+    if q < 1:
+        return abs(-6 * s * (1 - q) ** 2)
+    return 0.0
+
+
+# ---- tables (sph_funcs.py:51-118) ---------------------------------------
+@pytest.mark.parametrize("h,n", [(0.04, 1000), (0.1, 8192)])
+def test_tables_match_closed_form(h, n):
+    wt, gt = tables.make_tables(h, n, n)
+    r = np.linspace(0, h, n)
+    w_ref = np.array([w_exact(x, h) for x in r])
+    assert np.allclose(wt, w_ref, rtol=1e-6, atol=1e-6 * sigma(h))
+    g_ref = np.array([dw_exact(x, h) / x if x > 1e-10 else 0.0 for x in r])
+    assert np.allclose(gt[1:], g_ref[1:], rtol=1e-6, atol=1e-6 * sigma(h) / h ** 2)
+    assert gt[0] == 0.0
+    assert wt[0] == np.float32(sigma(h))
+    assert wt[-1] == 0.0
+
+
+def test_table_normalisation():
+    """The tabulated W integrates to 1 over the support (4 pi r^2 dr)."""
+    h = 0.04
+    wt, _ = tables.make_tables(h, 1000, 1000)
+    r = np.linspace(0, h, 1000)
+    integral = np.trapezoid(4 * np.pi * r ** 2 * wt.astype(np.float64), r)
+    assert abs(integral - 1.0) < 1e-4
+
+
+# ---- hash keys (warp.HashGrid restated) ------------------------------------
+def test_hash_keys_known_values(orc):
+    dims = (6, 6, 6)
+    w = 0.04
+    pts = np.array([[0.0, 0.0, 0.0], [-0.02, 0.0, 0.0], [-0.05, 0, 0], [0.04, 0.0, 0.0],
+                    [0.0399, 0.08, 0.5], [1.0, 1.0, 1.0]], np.float32)
+    # cell = trunc(p / w) + 2^20, modulo dims; -0.02/0.04 = -0.5 truncates to 0
+    expect = []
+    for p in pts:
+        c = [(int(np.float32(v) * (np.float32(1.0) / np.float32(w))) + (1 << 20)) % d
+             for v, d in zip(p, dims)]
+        expect.append(c[2] * 36 + c[1] * 6 + c[0])
+    got = orc.hash_keys(pts, w, dims)
+    assert got.tolist() == expect
+    assert got[0] == got[1]  # trunc toward zero merges (-w, w) into one cell
+
+
+def test_hash_keys_random(orc):
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1.5, 2.5, (5000, 3)).astype(np.float32)
+    dims = (25, 12, 12)
+    inv = np.float32(1.0) / np.float32(0.02)
+    c = np.trunc(x * inv).astype(np.int64) + (1 << 20)
+    c = np.maximum(c, 0) % np.array(dims)
+    ref = c[:, 2] * dims[0] * dims[1] + c[:, 1] * dims[0] + c[:, 0]
+    assert np.array_equal(orc.hash_keys(x, 0.02, dims), ref)
+
+
+# ---- neighbour sets (IISPH.py:52-163) ----------------------------------------
+@pytest.mark.parametrize("dims", [(6, 6, 6), (3, 3, 3), (40, 40, 40)])
+def test_neighbor_sets_equal_brute_force(orc, dims):
+    h = 0.04
+    rng = np.random.default_rng(7)
+    # a lattice whose axis pairs sit exactly at distance h, a jittered cloud and
+    # points with negative coordinates
+    lat = scene.lattice_block((8, 8, 8), 0.02, (-0.06, -0.02, 0.01)).astype(np.float32)
+    cloud = rng.uniform(-0.1, 0.2, (600, 3)).astype(np.float32)
+    x = np.concatenate([lat, cloud])
+    cnt, idx, dist = orc.neighbors(x, x, h, dims, True, cap=256)
+    bf = brute_force_lists(x, x, h, True)
+    for i in range(x.shape[0]):
+        got = idx[i, :cnt[i]].tolist()
+        assert len(got) == len(set(got))
+        assert set(got) == bf[i], i
+    # symmetry of the fluid-fluid relation
+    pairs = {(i, j) for i in range(x.shape[0]) for j in idx[i, :cnt[i]]}
+    assert all((j, i) in pairs for (i, j) in pairs)
+
+
+def test_borderline_pairs_follow_float32_distance(orc):
+    """Lattice pairs at exactly distance h = 2d fall either side of r < h in
+    float32 depending on the rounding of the coordinates."""
+    d = 0.01
+    x = scene.lattice_block((40, 3, 3), d).astype(np.float32)
+    cnt, idx, _ = orc.neighbors(x, x, 2 * d, (100, 100, 100), True, cap=64)
+    bf = brute_force_lists(x, x, 2 * d, True)
+    assert [set(idx[i, :cnt[i]].tolist()) for i in range(len(x))] == bf
+    inside = sum(1 for i in range(38) if (3 * 3 * (i + 2)) in bf[9 * i])
+    assert 0 < inside < 38  # some exactly-h axis pairs are in, some out
+
+
+def test_neighbor_order_is_cell_order(orc):
+    """Order: cells with x innermost, then y, then z over the query range;
+    ascending index inside a cell (hash_grid_query)."""
+    h = 0.04
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0, 0.3, (400, 3)).astype(np.float32)
+    dims = (20, 20, 20)
+    cnt, idx, _ = orc.neighbors(x, x, h, dims, True, cap=128)
+    inv = np.float32(1.0) / np.float32(h)
+    cell = np.trunc(x * inv).astype(np.int64)
+    hf = np.float32(h)
+    for i in range(0, 400, 7):
+        p = x[i]
+        lo = np.trunc((p - hf) * inv).astype(np.int64)
+        hi = np.trunc((p + hf) * inv).astype(np.int64)
+        order = []
+        for cz in range(lo[2], hi[2] + 1):
+            for cy in range(lo[1], hi[1] + 1):
+                for cx in range(lo[0], hi[0] + 1):
+                    members = np.nonzero((cell[:, 0] == cx) & (cell[:, 1] == cy) & (cell[:, 2] == cz))[0]
+                    for j in members:
+                        dd = p - x[j]
+                        r = np.sqrt((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2])
+                        if j != i and r < hf:
+                            order.append(int(j))
+        assert idx[i, :cnt[i]].tolist() == order
+
+
+# ---- closed forms on an interior lattice -----------------------------------
+def _interior_lattice_sum():
+    """sum_j W(r_ij)/sigma over an infinite cubic lattice with h = 2d (self
+    included): 1 + 6/4 + 12*2(1 - 1/sqrt2)^3 + 8*2(1 - sqrt3/2)^3."""
+    return 1 + 6 * 0.25 + 12 * 2 * (1 - 1 / math.sqrt(2)) ** 3 + 8 * 2 * (1 - math.sqrt(3) / 2) ** 3
+
+
+def test_interior_density_closed_form(orc):
+    cfg = config.dambreak_8k().replace(fluid_cells=(12, 12, 12), xsph=0.0, force_iters=2)
+    sc, P, wt, gt = oracle_setup(cfg)
+    psi = orc.boundary_psi(P, wt, sc.xb)
+    _, _, info = orc.step(P, wt, gt, sc.x, np.zeros_like(sc.x), sc.xb, sc.vb, psi)
+    rho_exact = cfg.rho0 * _interior_lattice_sum() / math.pi  # = 0.99997247 rho0
+    assert abs(rho_exact / cfg.rho0 - 0.99997247) < 1e-8
+    k = (6 * 12 + 6) * 12 + 6
+    assert abs(info["rho"][k] / rho_exact - 1) < 1e-5
+    # Sum_j grad W = 0 in the interior: d_ii = 0 and rho_adv = rho at rest
+    assert np.all(np.abs(info["d_ii"][k]) < 1e-12)
+    # a_ii = -(rho0/rho)^2 (dt/d)^2 * sum |W'|^2 h^2 / (4 pi^2 sigma^2 h^2 ...)
+    s = (6 * 1.5 ** 2 + 12 * (6 * (1 - 1 / math.sqrt(2)) ** 2) ** 2
+         + 8 * (6 * (1 - math.sqrt(3) / 2) ** 2) ** 2) / (4 * math.pi ** 2)
+    assert abs(s - 0.42483995) < 1e-7
+    a_exact = -(cfg.rho0 / info["rho"][k]) ** 2 * (cfg.dt / cfg.spacing) ** 2 * s
+    assert abs(info["a_ii"][k] / a_exact - 1) < 1e-4
+
+
+def test_planar_wall_psi_closed_form(orc):
+    cfg = config.dambreak_8k()
+    sc, P, wt, _ = oracle_setup(cfg)
+    psi = orc.boundary_psi(P, wt, sc.xb)
+    # interior point of a one-layer plane: psi = rho0 / (sigma (1 + 4/4 + 4*2(1-1/sqrt2)^3))
+    s = sigma(cfg.h) * (1 + 1 + 4 * 2 * (1 - 1 / math.sqrt(2)) ** 3)
+    psi_exact = cfg.rho0 / s
+    assert abs(psi_exact / (cfg.rho0 * cfg.spacing ** 3) - 1.4273413) < 1e-6
+    k = int(np.argmin(np.linalg.norm(sc.xb - np.array([0.5, -0.02, 0.5], np.float32), axis=1)))
+    assert abs(psi[k] / psi_exact - 1) < 1e-5
+
+
+# ---- the reference's own invariant: check_rho_error (IISPH.py:570-648) -------
+def _csr(orc, xq, xs, h, dims, excl):
+    cnt, idx, _ = orc.neighbors(xq, xs, h, dims, excl, cap=128)
+    return cnt, idx
+
+
+def test_check_rho_error_identity(orc):
+    """rho_adv + A p equals rho_adv + dt div(dv_p), with dv_p the pressure
+    velocity change dt (m sum -(p_i/rho_i^2 + p_j/rho_j^2) gradW_ij - sum psi_b
+    p_i/rho_i^2 gradW_ib), evaluated here in float64."""
+    cfg = config.dambreak_8k().replace(fluid_cells=(10, 10, 10), force_iters=5)
+    sc, P, wt, gt = oracle_setup(cfg)
+    psi = orc.boundary_psi(P, wt, sc.xb)
+    x0 = sc.x.copy()
+    _, _, info = orc.step(P, wt, gt, x0, np.zeros_like(x0), sc.xb, sc.vb, psi)
+    dims = tuple(P.grid_dims)
+    cf, jf = _csr(orc, x0, x0, cfg.h, dims, True)
+    cb, jb = _csr(orc, x0, sc.xb, cfg.h, tuple(P.bgrid_dims), False)
+    X = x0.astype(np.float64)
+    XB = sc.xb.astype(np.float64)
+    p = info["p"].astype(np.float64)
+    rho = info["rho"].astype(np.float64)
+    m, dt, h = cfg.mass, cfg.dt, cfg.h
+
+    def gradw(r):
+        l = np.linalg.norm(r)
+        return dw_exact(l, h) / l * r if 1e-10 < l < h else np.zeros(3)
+
+    n = X.shape[0]
+    dvp = np.zeros((n, 3))
+    for i in range(n):
+        t1 = np.zeros(3)
+        for j in jf[i, :cf[i]]:
+            t1 -= (p[i] / rho[i] ** 2 + p[j] / rho[j] ** 2) * gradw(X[j] - X[i])
+        t2 = np.zeros(3)
+        for b in jb[i, :cb[i]]:
+            t2 -= p[i] / rho[i] ** 2 * gradw(XB[b] - X[i]) * psi[b]
+        dvp[i] = (t1 * m + t2) * dt
+    err = 0.0
+    scale = 0.0
+    for i in range(n):
+        t1 = sum(np.dot(dvp[i] - dvp[j], gradw(X[j] - X[i])) for j in jf[i, :cf[i]])
+        t2 = sum(np.dot(dvp[i], gradw(XB[b] - X[i])) * psi[b] for b in jb[i, :cb[i]])
+        div = (t1 * m + t2) * dt
+        err = max(err, abs(div - info["Ap"][i]))
+        scale = max(scale, abs(div))
+    assert scale > 1.0  # pressures are active
+    # float32 table interpolation vs exact derivative and float32 sums
+    assert err < 2e-3 * scale, (err, scale)
+
+
+def test_pressure_impulse_conserves_momentum(orc):
+    """Without boundaries the pressure velocity change (d_ii p + s)/dt sums
+    to zero over the fluid: each pair acts equal and opposite."""
+    cfg = config.dambreak_8k().replace(gravity=0.0, xsph=0.0, force_iters=4)
+    d = cfg.spacing
+    x = scene.lattice_block((8, 8, 8), 0.9 * d).astype(np.float32)
+    sc, P, wt, gt = oracle_setup(cfg, scene.Scene(x, np.zeros((0, 3), np.float32),
+                                                   np.zeros((0, 3), np.float32), 0,
+                                                   x.min(0).astype(np.float64),
+                                                   x.max(0).astype(np.float64)))
+    psi = np.zeros(0, np.float32)
+    _, v1, info = orc.step(P, wt, gt, x, np.zeros_like(x), sc.xb, sc.vb, psi)
+    assert info["p"].max() > 0
+    dv = (v1 - info["v_adv"]).astype(np.float64)
+    total = np.abs(dv.sum(axis=0)).max()
+    assert total < 1e-5 * np.abs(dv).sum()
+
+
+# ---- Jacobi loop bounds (IISPH.py:1200-1207) -----------------------------------
+@pytest.mark.parametrize("eta,expect", [(10.0, 2), (0.0, 101)])
+def test_jacobi_iteration_bounds(orc, eta, expect):
+    cfg = config.dambreak_8k().replace(fluid_cells=(8, 8, 8), eta=eta)
+    sc, P, wt, gt = oracle_setup(cfg)
+    psi = orc.boundary_psi(P, wt, sc.xb)
+    _, _, info = orc.step(P, wt, gt, sc.x, np.zeros_like(sc.x), sc.xb, sc.vb, psi)
+    assert info["iterations"] == expect
+
+
+def test_fixed_point_residual_tracks_float_sum(orc):
+    """The order-independent fixed-point residual and the sequential float32
+    sum of a one-thread run agree to float32 accuracy."""
+    cfg = config.dambreak_8k().replace(force_iters=6)
+    sc, P, wt, gt = oracle_setup(cfg)
+    psi = orc.boundary_psi(P, wt, sc.xb)
+    v = np.zeros_like(sc.x)
+    _, _, a = orc.step(P, wt, gt, sc.x, v, sc.xb, sc.vb, psi, residual_mode=0)
+    _, _, b = orc.step(P, wt, gt, sc.x, v, sc.xb, sc.vb, psi, residual_mode=1)
+    assert np.allclose(a["res_hist"], b["res_hist"], rtol=1e-4)

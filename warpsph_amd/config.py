@@ -1,0 +1,171 @@
+"""Solver configuration.
+
+The reference freezes its physics as module globals in wp_sph/main.py:25-61
+and in IISPH.__init__ (IISPH.py:742-749, 763-766, 771-780). ``SolverConfig``
+holds the same quantities explicitly; its defaults are the reference's
+values, and named presets describe the benchmark workloads.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from dataclasses import dataclass, field
+from typing import Optional, Tuple
+
+from . import tables
+
+DT_CFL = 0
+DT_FIXED = 1
+WARM_ZERO = 0
+WARM_PREVIOUS = 1
+WARM_EXTERNAL = 2
+JACOBI_HOST = 0
+JACOBI_PERSISTENT = 1
+LAYOUT_LINEAR = 0
+LAYOUT_MORTON = 1
+
+
+@dataclass
+class SolverConfig:
+    # physics (wp_sph/main.py:25-31, 57-61)
+    spacing: float = 0.05          # DIAMETER
+    rho0: float = 1.0e3            # RHO_0
+    gravity: float = 9.8           # GRAVITY (acts along -y)
+    omega: float = 0.5             # OMEGA
+    eta: float = 1.0e-3            # ETA
+    inv_small: float = 1.0e-6      # INV_SMALL
+    vis_mu: float = 0.0            # VIS_MU (Laplacian viscosity, IISPH.py:273-278)
+    xsph: float = 0.0              # XSPH velocity smoothing coefficient
+    fps: int = 60                  # FPS
+    # time step (IISPH.py:747, 1368-1370)
+    dt_mode: int = DT_CFL
+    dt: Optional[float] = None     # None -> TIME_STEP_MAX
+    cfl: float = 0.4
+    # Jacobi loop bounds (IISPH.py:1200-1207)
+    min_iters: int = 2
+    max_iters: int = 101
+    force_iters: int = 0
+    warm_start: int = WARM_ZERO
+    # kernel tables (sph_funcs.py:43-44)
+    w_size: int = tables.W_SIZE
+    grad_size: int = tables.GRAD_W_SIZE
+    # scene (wp_sph/main.py:48-51, IISPH.py:749, 763-766)
+    box: Tuple[float, float, float] = (4.0, 4.0, 4.0)
+    boundary_layers: int = 3
+    scene: str = "reference_box"   # generator name in warpsph_amd.scene
+    fluid_min: Optional[Tuple[float, float, float]] = None
+    fluid_max: Optional[Tuple[float, float, float]] = None
+    fluid_cells: Optional[Tuple[int, int, int]] = None
+    fluid_origin: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    jitter: float = 0.0            # fraction of spacing
+    seed: int = 0
+    steps: int = 5000
+    # device layout
+    grid: str = "domain"           # "domain" (covering) or "reference" (int(BOX/h/4))
+    cell_layout: int = LAYOUT_MORTON
+    max_ff: int = 64
+    max_fs: int = 64
+    jacobi_mode: int = JACOBI_PERSISTENT
+    block: int = 256
+    coop_blocks_per_cu: int = 0
+    extra: dict = field(default_factory=dict)
+
+    # ---- derived quantities (wp_sph/main.py:57-60) ---------------------
+    @property
+    def h(self) -> float:
+        """SMOOTHING_LENGTH = 2 DIAMETER."""
+        return 2 * self.spacing
+
+    @property
+    def mass(self) -> float:
+        """FLUID_MASS = RHO_0 DIAMETER^3."""
+        return self.rho0 * self.spacing ** 3
+
+    @property
+    def sigma(self) -> float:
+        """SIGMA = 8 / (pi h^3)."""
+        return 8 / (math.pi * self.h ** 3)
+
+    @property
+    def dt_max(self) -> float:
+        """TIME_STEP_MAX = 0.05 DIAMETER."""
+        return 5e-2 * self.spacing
+
+    @property
+    def dt0(self) -> float:
+        return self.dt if self.dt is not None else self.dt_max
+
+    def reference_grid_dims(self) -> Tuple[int, int, int]:
+        """Hash dims of the reference, int(BOX / h / 4) (IISPH.py:771-780)."""
+        return tuple(int(b / self.h / 4) for b in self.box)  # type: ignore
+
+    def replace(self, **kw) -> "SolverConfig":
+        return dataclasses.replace(self, **kw)
+
+
+def reference() -> SolverConfig:
+    """The reference's own scene: SceneType.EMPTY, 4x4x4 m box, d = 0.05,
+    3 wall layers, fluid block from (0.02,0.02,0.02) to (0.98,0.4,0.55) of the
+    box, CFL dt, 8192/65536-sample tables, p0 = 0 (main.py:19-61,
+    IISPH.py:738-766)."""
+    return SolverConfig()
+
+
+def dambreak_8k() -> SolverConfig:
+    """20x20x20 = 8,000 fluid particles at 0.02 m, block corner at the origin
+    of a 1x1x1 m box with one wall layer; g = 9.81, dt = 1e-3 fixed, XSPH 0.01,
+    1000-sample tables; 200 steps."""
+    return SolverConfig(
+        spacing=0.02, gravity=9.81, xsph=0.01, dt_mode=DT_FIXED, dt=1e-3,
+        w_size=1000, grad_size=1000, box=(1.0, 1.0, 1.0), boundary_layers=1,
+        scene="dam_break", fluid_cells=(20, 20, 20), steps=200)
+
+
+def dambreak_1m() -> SolverConfig:
+    """100x100x100 = 1,000,000 fluid particles at 0.01 m in a 2x1x1 m box with
+    one wall layer, positions jittered by +-1% of the spacing from seed 0;
+    otherwise as dambreak_8k; 500 steps."""
+    return dambreak_8k().replace(
+        spacing=0.01, box=(2.0, 1.0, 1.0), fluid_cells=(100, 100, 100),
+        jitter=0.01, seed=0, steps=500)
+
+
+def torus_4m() -> SolverConfig:
+    """~4M fluid particles at 0.005 m falling onto a torus (R = 0.3, r = 0.1)
+    and a ground plane, both Poisson-disk sampled at 0.005 m from seed 1;
+    300 steps."""
+    return dambreak_8k().replace(
+        spacing=0.005, box=(1.2, 1.6, 1.2), boundary_layers=0,
+        scene="torus", fluid_cells=(160, 160, 160), seed=1, steps=300,
+        extra={"torus_R": 0.3, "torus_r": 0.1})
+
+
+def batch_256() -> SolverConfig:
+    """256 independent dam-break scenes, fluid counts log-uniform in
+    2k..64k, random block aspect ratios and box sizes 0.5..2 m from seed 2,
+    spacing 0.02 m; 300 steps each."""
+    return dambreak_8k().replace(scene="batch", seed=2, steps=300,
+                                 extra={"n_scenes": 256, "n_min": 2000,
+                                        "n_max": 64000})
+
+
+def learned_1m() -> SolverConfig:
+    """dambreak_1m with the learned pressure start value; 500 steps."""
+    return dambreak_1m().replace(warm_start=WARM_EXTERNAL)
+
+
+PRESETS = {
+    "reference": reference,
+    "dambreak_8k": dambreak_8k,
+    "dambreak_1m": dambreak_1m,
+    "torus_4m": torus_4m,
+    "batch_256": batch_256,
+    "learned_1m": learned_1m,
+}
+
+
+def preset(name: str) -> SolverConfig:
+    try:
+        return PRESETS[name]()
+    except KeyError:
+        raise ValueError(f"unknown preset {name!r}; known: {sorted(PRESETS)}") from None

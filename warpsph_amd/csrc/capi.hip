@@ -1,0 +1,753 @@
+// capi.hip -- extern "C" entry points of libwsph.so (declared in include/wsph.h).
+//
+// The solver object owns every device buffer (allocated in wsph_set_*,
+// released in wsph_destroy) and one HIP stream at default priority. Particle
+// state lives in cell-sorted order; wsph_get_field scatters it back to the
+// caller's original order.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "kernels.h"
+
+using namespace wsph;
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(x)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) return fail(WSPH_E_HIP, "%s: %s", #x, hipGetErrorString(e_));     \
+  } while (0)
+
+template <typename T>
+static hipError_t dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void **)p, sizeof(T) * count);
+}
+template <typename T>
+static void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+// ---- helper kernels ---------------------------------------------------------
+__global__ void k_pack3(const float *__restrict__ x3, int n, float4 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = x3 ? make_float4(x3[3 * (long)i], x3[3 * (long)i + 1], x3[3 * (long)i + 2], 0.0f)
+              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+__global__ void k_iota(int *__restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+__global__ void k_fill_f32(float *__restrict__ p, int n, float v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_gather_boundary(const int *__restrict__ perm, int n, const float4 *__restrict__ a,
+                                  const float4 *__restrict__ b, float4 *__restrict__ ao,
+                                  float4 *__restrict__ bo, int *__restrict__ orig_o) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const int i = perm[f];
+  ao[f] = a[i];
+  bo[f] = b[i];
+  orig_o[f] = i;
+}
+__global__ void k_gather_by_orig(const float *__restrict__ src_orig, const int *__restrict__ orig,
+                                 int n, float *__restrict__ out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < n) out[f] = src_orig[orig[f]];
+}
+__global__ void k_keys4(const float4 *__restrict__ pos, int n, float inv, int d0, int d1, int d2,
+                        int *__restrict__ keys) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int dims[3] = {d0, d1, d2};
+  const float4 p = pos[i];
+  keys[i] = linear_key(p.x, p.y, p.z, inv, dims);
+}
+__global__ void k_export_lists(const int *__restrict__ nbr, const int *__restrict__ cnt, int n,
+                               const int *__restrict__ orig, const int *__restrict__ other_orig,
+                               int cap_in, int cap_out, int *__restrict__ cnt_o,
+                               int *__restrict__ idx_o) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long o = orig[i];
+  const int c = cnt[i];
+  cnt_o[o] = c;
+  for (int k = 0; k < c && k < cap_out; ++k) idx_o[o * cap_out + k] = other_orig[nbr[(long)k * n + i]];
+}
+__global__ void k_solve_result(DevState *st, int iters, double residual, int converged,
+                               int ap_evals) {
+  st->iterations = iters;
+  st->residual = residual;
+  st->converged = converged;
+  st->ap_evals += ap_evals;
+}
+__global__ void k_add_ap_evals(DevState *st, int k) { st->ap_evals += k; }
+__global__ void k_set_dt(DevState *st, double dt, double t) {
+  st->dt = dt;
+  st->sim_time = t;
+}
+__global__ void k_accumulate_totals(DevState *st, long long *tot) {
+  tot[0] += 1;                  // steps
+  tot[1] += st->iterations;     // Jacobi iterations
+  tot[2] += st->converged ? 0 : 1;
+  tot[3] += st->ap_evals;
+  tot[4] |= st->status;
+}
+
+static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---- solver object ------------------------------------------------------------
+struct GridBufs {
+  long ncell = 0;
+  unsigned *count = nullptr, *start = nullptr;
+  std::vector<unsigned *> scan;
+  int dims[3] = {0, 0, 0};
+};
+
+struct wsph_solver {
+  wsph_params P;
+  SphConst C;
+  hipStream_t stream = nullptr;
+  int n = 0, nb = 0, cap_n = 0;
+  float *wt = nullptr, *gt = nullptr;
+  bool lds_tables = false;
+  DevState *st = nullptr;
+  DevState *st_host = nullptr;   // pinned mirror
+  long long *totals = nullptr;   // device counters
+  // fluid (double-buffered across the per-step reorder)
+  float4 *pos[2] = {nullptr, nullptr}, *vel[2] = {nullptr, nullptr};
+  float *pprev[2] = {nullptr, nullptr};
+  int *orig[2] = {nullptr, nullptr};
+  int cur = 0;
+  float4 *vadv = nullptr, *dp[2] = {nullptr, nullptr}, *sap = nullptr, *aux = nullptr;
+  float *ap = nullptr, *pstart = nullptr, *p_ext = nullptr;
+  bool have_p_ext = false;
+  int *slot = nullptr, *rank = nullptr, *tmp = nullptr, *perm = nullptr;
+  int *nbr_ff = nullptr, *cnt_ff = nullptr, *nbr_fs = nullptr, *cnt_fs = nullptr;
+  GridBufs fg, bg;
+  // boundary
+  float4 *bpos = nullptr, *bvel = nullptr;
+  int *borig = nullptr;
+  int final_dp = 0;
+  bool searched = false, advected = false, solved = false;
+  // timing: a ring of per-step event sets, read back lazily so that timed
+  // steps need no host synchronisation
+  bool timing = false;
+  static constexpr int kRing = 64, kEv = 5;
+  hipEvent_t ev[kRing * kEv] = {};
+  long t_recorded = 0, t_processed = 0;
+  double t_sum[kEv] = {0};
+  int persistent_grid = 0;
+};
+
+static long grid_cells(const int *dims, int layout) {
+  if (layout == WSPH_LAYOUT_MORTON) return (long)morton3(dims[0] - 1, dims[1] - 1, dims[2] - 1) + 1;
+  return (long)dims[0] * dims[1] * dims[2];
+}
+
+static int alloc_grid(GridBufs &g, const int *dims, int layout) {
+  for (auto p : g.scan) dfree(p);
+  g.scan.clear();
+  dfree(g.count);
+  dfree(g.start);
+  memcpy(g.dims, dims, sizeof(g.dims));
+  g.ncell = grid_cells(dims, layout);
+  HIP_TRY(dalloc(&g.count, g.ncell));
+  HIP_TRY(dalloc(&g.start, g.ncell));
+  long sizes[8];
+  const int L = scan_levels_needed(g.ncell, sizes, 8);
+  for (int l = 0; l < L; ++l)
+    for (int r = 0; r < 2; ++r) {
+      unsigned *p = nullptr;
+      HIP_TRY(dalloc(&p, sizes[l]));
+      g.scan.push_back(p);
+    }
+  g.scan.push_back(nullptr);
+  g.scan.push_back(nullptr);
+  return WSPH_OK;
+}
+
+static Bufs make_bufs(wsph_solver *s) {
+  Bufs b;
+  memset(&b, 0, sizeof(b));
+  b.n = s->n;
+  b.nb = s->nb;
+  b.pos = s->pos[s->cur];
+  b.vel = s->vel[s->cur];
+  b.vadv = s->vadv;
+  b.dp[0] = s->dp[0];
+  b.dp[1] = s->dp[1];
+  b.sap = s->sap;
+  b.aux = s->aux;
+  b.ap = s->ap;
+  b.pprev = s->pprev[s->cur];
+  b.pstart = s->pstart;
+  b.fstart = s->fg.start;
+  b.fcount = s->fg.count;
+  b.nbr_ff = s->nbr_ff;
+  b.cnt_ff = s->cnt_ff;
+  b.nbr_fs = s->nbr_fs;
+  b.cnt_fs = s->cnt_fs;
+  b.cap_ff = s->P.max_ff;
+  b.cap_fs = s->P.max_fs;
+  b.bpos = s->bpos;
+  b.bvel = s->bvel;
+  b.bstart = s->bg.start;
+  b.bcount = s->bg.count;
+  b.wt = s->wt;
+  b.gt = s->gt;
+  b.lds_tables = s->lds_tables;
+  b.st = s->st;
+  return b;
+}
+
+static int check_dims(const int *d, const char *what) {
+  for (int k = 0; k < 3; ++k)
+    if (d[k] < 1 || d[k] > 1024) return fail(WSPH_E_ARG, "%s dims must be in [1, 1024]", what);
+  return WSPH_OK;
+}
+
+extern "C" {
+
+const char *wsph_last_error(void) { return g_err; }
+int wsph_abi_version(void) { return WSPH_ABI_VERSION; }
+
+int wsph_create(const wsph_params *params, wsph_solver **out) {
+  if (!params || !out) return fail(WSPH_E_ARG, "null argument");
+  const wsph_params &P = *params;
+  if (!(P.h > 0 && P.spacing > 0 && P.rho0 > 0 && P.mass > 0 && P.dt > 0))
+    return fail(WSPH_E_ARG, "h, spacing, rho0, mass and dt must be positive");
+  if (P.max_iters < 1 || P.max_iters > kMaxIters || P.force_iters < 0 ||
+      P.force_iters > kMaxIters || P.min_iters < 0)
+    return fail(WSPH_E_ARG, "iteration bounds must satisfy 1 <= max_iters <= %d", kMaxIters);
+  if (P.max_ff < 1 || P.max_fs < 1) return fail(WSPH_E_ARG, "neighbour capacities must be >= 1");
+  if (P.block < 64 || P.block > 1024 || (P.block & 63))
+    return fail(WSPH_E_ARG, "block must be a multiple of 64 in [64, 1024]");
+  if (P.cell_layout != WSPH_LAYOUT_LINEAR && P.cell_layout != WSPH_LAYOUT_MORTON)
+    return fail(WSPH_E_ARG, "unknown cell layout");
+  if (P.warm_start < WSPH_WARM_ZERO || P.warm_start > WSPH_WARM_EXTERNAL)
+    return fail(WSPH_E_ARG, "unknown warm start mode");
+  int rc;
+  if ((rc = check_dims(P.grid_dims, "grid")) || (rc = check_dims(P.bgrid_dims, "boundary grid")))
+    return rc;
+  wsph_solver *s = new wsph_solver();
+  s->P = P;
+  SphConst &C = s->C;
+  memset(&C, 0, sizeof(C));
+  C.h = (float)P.h;
+  C.cell_inv = 1.0f / C.h;
+  C.mass = (float)P.mass;
+  C.rho0 = (float)P.rho0;
+  C.omega = (float)P.omega;
+  C.inv_small = (float)P.inv_small;
+  C.g_y = -(float)P.gravity;
+  C.vis_mu = (float)P.vis_mu;
+  C.xsph = (float)P.xsph;
+  memcpy(C.fdims, P.grid_dims, sizeof(C.fdims));
+  memcpy(C.bdims, P.bgrid_dims, sizeof(C.bdims));
+  C.layout = P.cell_layout;
+  *out = nullptr;
+#define CREATE_TRY(x)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) {                                                                  \
+      int r_ = fail(WSPH_E_HIP, "%s: %s", #x, hipGetErrorString(e_));                        \
+      wsph_destroy(s);                                                                       \
+      return r_;                                                                             \
+    }                                                                                        \
+  } while (0)
+  CREATE_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  CREATE_TRY(dalloc(&s->st, 1));
+  CREATE_TRY(hipHostMalloc((void **)&s->st_host, sizeof(DevState), hipHostMallocDefault));
+  CREATE_TRY(dalloc(&s->totals, 8));
+  CREATE_TRY(hipMemsetAsync(s->totals, 0, sizeof(long long) * 8, s->stream));
+  CREATE_TRY(hipMemsetAsync(s->st, 0, sizeof(DevState), s->stream));
+  for (auto &e : s->ev) CREATE_TRY(hipEventCreate(&e));
+  k_set_dt<<<1, 1, 0, s->stream>>>(s->st, P.dt, 0.0);
+  CREATE_TRY(hipGetLastError());
+  int rcg = alloc_grid(s->fg, P.grid_dims, P.cell_layout);
+  if (!rcg) rcg = alloc_grid(s->bg, P.bgrid_dims, P.cell_layout);
+  if (rcg) {
+    wsph_destroy(s);
+    return rcg;
+  }
+  CREATE_TRY(hipStreamSynchronize(s->stream));
+#undef CREATE_TRY
+  *out = s;
+  return WSPH_OK;
+}
+
+static void free_fluid(wsph_solver *s) {
+  for (int k = 0; k < 2; ++k) {
+    dfree(s->pos[k]);
+    dfree(s->vel[k]);
+    dfree(s->pprev[k]);
+    dfree(s->orig[k]);
+    dfree(s->dp[k]);
+  }
+  dfree(s->vadv);
+  dfree(s->sap);
+  dfree(s->aux);
+  dfree(s->ap);
+  dfree(s->pstart);
+  dfree(s->p_ext);
+  dfree(s->slot);
+  dfree(s->rank);
+  dfree(s->tmp);
+  dfree(s->perm);
+  dfree(s->nbr_ff);
+  dfree(s->cnt_ff);
+  dfree(s->nbr_fs);
+  dfree(s->cnt_fs);
+  s->n = 0;
+}
+
+int wsph_destroy(wsph_solver *s) {
+  if (!s) return WSPH_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  free_fluid(s);
+  dfree(s->bpos);
+  dfree(s->bvel);
+  dfree(s->borig);
+  dfree(s->wt);
+  dfree(s->gt);
+  dfree(s->st);
+  dfree(s->totals);
+  if (s->st_host) (void)hipHostFree(s->st_host);
+  for (GridBufs *g : {&s->fg, &s->bg}) {
+    dfree(g->count);
+    dfree(g->start);
+    for (auto p : g->scan) dfree(p);
+  }
+  for (auto &e : s->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return WSPH_OK;
+}
+
+int wsph_set_tables(wsph_solver *s, const float *w_table, int w_size, const float *grad_table,
+                    int grad_size) {
+  if (!s || !w_table || !grad_table || w_size < 2 || grad_size < 2)
+    return fail(WSPH_E_ARG, "tables need at least 2 samples");
+  dfree(s->wt);
+  dfree(s->gt);
+  HIP_TRY(dalloc(&s->wt, w_size));
+  HIP_TRY(dalloc(&s->gt, grad_size));
+  HIP_TRY(hipMemcpyAsync(s->wt, w_table, sizeof(float) * w_size, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->gt, grad_table, sizeof(float) * grad_size, hipMemcpyHostToDevice,
+                         s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->C.nw = w_size;
+  s->C.ng = grad_size;
+  const float h_inv = (float)(1.0 / s->P.h);
+  s->C.w_scale = (float)(w_size - 1) * h_inv;
+  s->C.g_scale = (float)(grad_size - 1) * h_inv;
+  s->lds_tables = (w_size + grad_size) <= kMaxLdsTable;
+  return WSPH_OK;
+}
+
+// Sort points (float4, current order = input order) into cell order.
+static int sort_points(wsph_solver *s, GridBufs &g, const float4 *pos, const int *orig, int n,
+                       int *slot, int *rank, int *tmp, int *perm) {
+  GridArgs a;
+  a.n = n;
+  a.ncell = g.ncell;
+  a.inv = s->C.cell_inv;
+  memcpy(a.dims, g.dims, sizeof(a.dims));
+  a.layout = s->P.cell_layout;
+  a.pos = pos;
+  a.orig = orig;
+  a.slot = slot;
+  a.rank = rank;
+  a.tmp = tmp;
+  a.perm = perm;
+  a.count = g.count;
+  a.start = g.start;
+  a.scan_bufs = g.scan.data();
+  sort_into_cells(a, s->stream);
+  HIP_TRY(hipGetLastError());
+  return WSPH_OK;
+}
+
+int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) {
+  if (!s || nb < 0 || (nb > 0 && !xb)) return fail(WSPH_E_ARG, "bad boundary arguments");
+  if (!s->wt) return fail(WSPH_E_ARG, "call wsph_set_tables first");
+  dfree(s->bpos);
+  dfree(s->bvel);
+  dfree(s->borig);
+  s->nb = nb;
+  HIP_TRY(dalloc(&s->bpos, nb));
+  HIP_TRY(dalloc(&s->bvel, nb));
+  HIP_TRY(dalloc(&s->borig, nb));
+  float4 *p4 = nullptr, *v4 = nullptr;
+  int *id = nullptr, *sl = nullptr, *rk = nullptr, *tp = nullptr, *pm = nullptr;
+  int rc = WSPH_OK;
+  if (nb > 0) {
+    HIP_TRY(dalloc(&p4, nb));
+    HIP_TRY(dalloc(&v4, nb));
+    HIP_TRY(dalloc(&id, nb));
+    HIP_TRY(dalloc(&sl, nb));
+    HIP_TRY(dalloc(&rk, nb));
+    HIP_TRY(dalloc(&tp, nb));
+    HIP_TRY(dalloc(&pm, nb));
+    k_pack3<<<nblk(nb, 256), 256, 0, s->stream>>>(xb, nb, p4);
+    k_pack3<<<nblk(nb, 256), 256, 0, s->stream>>>(vb, nb, v4);
+    k_iota<<<nblk(nb, 256), 256, 0, s->stream>>>(id, nb);
+  }
+  rc = sort_points(s, s->bg, p4, id, nb, sl, rk, tp, pm);
+  if (!rc && nb > 0) {
+    k_gather_boundary<<<nblk(nb, 256), 256, 0, s->stream>>>(pm, nb, p4, v4, s->bpos, s->bvel,
+                                                            s->borig);
+    Bufs b = make_bufs(s);
+    launch_boundary_psi(b, s->C, s->stream);
+    if (hipGetLastError() != hipSuccess) rc = fail(WSPH_E_HIP, "boundary psi launch failed");
+  }
+  hipError_t e = hipStreamSynchronize(s->stream);
+  dfree(p4);
+  dfree(v4);
+  dfree(id);
+  dfree(sl);
+  dfree(rk);
+  dfree(tp);
+  dfree(pm);
+  if (e != hipSuccess) return fail(WSPH_E_HIP, "boundary setup: %s", hipGetErrorString(e));
+  return rc;
+}
+
+int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
+  if (!s || n < 0 || (n > 0 && !x)) return fail(WSPH_E_ARG, "bad fluid arguments");
+  if (!s->wt) return fail(WSPH_E_ARG, "call wsph_set_tables first");
+  if (!s->bpos) return fail(WSPH_E_ARG, "call wsph_set_boundary first");
+  if (n != s->cap_n) {
+    free_fluid(s);
+    for (int k = 0; k < 2; ++k) {
+      HIP_TRY(dalloc(&s->pos[k], n));
+      HIP_TRY(dalloc(&s->vel[k], n));
+      HIP_TRY(dalloc(&s->pprev[k], n));
+      HIP_TRY(dalloc(&s->orig[k], n));
+      HIP_TRY(dalloc(&s->dp[k], n));
+    }
+    HIP_TRY(dalloc(&s->vadv, n));
+    HIP_TRY(dalloc(&s->sap, n));
+    HIP_TRY(dalloc(&s->aux, n));
+    HIP_TRY(dalloc(&s->ap, n));
+    HIP_TRY(dalloc(&s->pstart, n));
+    HIP_TRY(dalloc(&s->p_ext, n));
+    HIP_TRY(dalloc(&s->slot, n));
+    HIP_TRY(dalloc(&s->rank, n));
+    HIP_TRY(dalloc(&s->tmp, n));
+    HIP_TRY(dalloc(&s->perm, n));
+    HIP_TRY(dalloc(&s->cnt_ff, n));
+    HIP_TRY(dalloc(&s->cnt_fs, n));
+    HIP_TRY(dalloc(&s->nbr_ff, (size_t)n * s->P.max_ff));
+    HIP_TRY(dalloc(&s->nbr_fs, (size_t)n * s->P.max_fs));
+    s->cap_n = n;
+  }
+  s->n = n;
+  s->cur = 0;
+  s->have_p_ext = false;
+  if (n > 0) {
+    k_pack3<<<nblk(n, 256), 256, 0, s->stream>>>(x, n, s->pos[0]);
+    k_pack3<<<nblk(n, 256), 256, 0, s->stream>>>(v, n, s->vel[0]);
+    k_iota<<<nblk(n, 256), 256, 0, s->stream>>>(s->orig[0], n);
+    k_fill_f32<<<nblk(n, 256), 256, 0, s->stream>>>(s->pprev[0], n, 0.0f);
+    k_fill_f32<<<nblk(n, 256), 256, 0, s->stream>>>(s->p_ext, n, 0.0f);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->searched = s->advected = s->solved = false;
+  return WSPH_OK;
+}
+
+int wsph_set_pressure(wsph_solver *s, const float *p) {
+  if (!s || !p) return fail(WSPH_E_ARG, "null argument");
+  if (s->n > 0)
+    HIP_TRY(hipMemcpyAsync(s->p_ext, p, sizeof(float) * s->n, hipMemcpyDeviceToDevice, s->stream));
+  s->have_p_ext = true;
+  return WSPH_OK;
+}
+
+static void rec(wsph_solver *s, int k) {
+  if (s->timing)
+    (void)hipEventRecord(s->ev[(s->t_recorded % wsph_solver::kRing) * wsph_solver::kEv + k],
+                         s->stream);
+}
+
+// Fold the oldest recorded step into the phase sums.
+static void timing_fold_one(wsph_solver *s) {
+  const int base = (int)(s->t_processed % wsph_solver::kRing) * wsph_solver::kEv;
+  (void)hipEventSynchronize(s->ev[base + 4]);
+  float ms;
+  for (int k = 0; k < 4; ++k) {
+    if (hipEventElapsedTime(&ms, s->ev[base + k], s->ev[base + k + 1]) == hipSuccess)
+      s->t_sum[k] += ms;
+  }
+  if (hipEventElapsedTime(&ms, s->ev[base], s->ev[base + 4]) == hipSuccess) s->t_sum[4] += ms;
+  s->t_processed++;
+}
+
+int wsph_neighbor_search(wsph_solver *s) {
+  if (!s || !s->pos[0]) return fail(WSPH_E_ARG, "no fluid set");
+  if (s->timing && s->t_recorded - s->t_processed >= wsph_solver::kRing) timing_fold_one(s);
+  launch_reset_state(make_bufs(s), s->stream);
+  rec(s, 0);
+  const int c = s->cur, o = c ^ 1;
+  int rc = sort_points(s, s->fg, s->pos[c], s->orig[c], s->n, s->slot, s->rank, s->tmp, s->perm);
+  if (rc) return rc;
+  gather_fluid(s->perm, s->n, s->pos[c], s->vel[c], s->pprev[c], s->orig[c], s->pos[o], s->vel[o],
+               s->pprev[o], s->orig[o], s->stream);
+  s->cur = o;
+  if (s->P.warm_start == WSPH_WARM_EXTERNAL && s->n > 0)
+    k_gather_by_orig<<<nblk(s->n, 256), 256, 0, s->stream>>>(s->p_ext, s->orig[o], s->n,
+                                                              s->pstart);
+  launch_lists(make_bufs(s), s->C, s->P.block, s->stream);
+  HIP_TRY(hipGetLastError());
+  s->searched = true;
+  return WSPH_OK;
+}
+
+int wsph_predict_advection(wsph_solver *s) {
+  if (!s || !s->searched) return fail(WSPH_E_ARG, "run wsph_neighbor_search first");
+  rec(s, 1);
+  const Bufs b = make_bufs(s);
+  const bool plain = s->P.vis_mu == 0.0 && s->P.xsph == 0.0;
+  launch_density(b, s->C, plain, s->P.block, s->stream);
+  if (!plain) launch_vadv(b, s->C, s->P.block, s->stream);
+  launch_assemble(b, s->C, s->P.warm_start, s->P.block, s->stream);
+  HIP_TRY(hipGetLastError());
+  s->advected = true;
+  return WSPH_OK;
+}
+
+int wsph_pressure_solve(wsph_solver *s, wsph_step_stats *stats) {
+  if (!s || !s->advected) return fail(WSPH_E_ARG, "run wsph_predict_advection first");
+  rec(s, 2);
+  const Bufs b = make_bufs(s);
+  const wsph_params &P = s->P;
+  int src = 0;
+  if (P.warm_start != WSPH_WARM_ZERO) {
+    // A p0 and the first update (IISPH.py:1155-1198, 1209-1219)
+    launch_j1(b, s->C, 0, P.block, s->stream);
+    launch_j2(b, s->C, 0, -1, P.block, s->stream);
+    k_add_ap_evals<<<1, 1, 0, s->stream>>>(s->st, 1);
+    src = 1;
+  }
+  if (P.jacobi_mode == WSPH_JACOBI_PERSISTENT) {
+    int per = 0;
+    int maxb = persistent_max_blocks(b, s->C, P.block, &per);
+    if (maxb <= 0) return fail(WSPH_E_UNSUPPORTED, "persistent kernel does not fit on the device");
+    if (P.coop_blocks_per_cu > 0 && P.coop_blocks_per_cu < per) maxb = maxb / per * P.coop_blocks_per_cu;
+    const int need = (int)nblk(s->n, P.block);
+    const int grid = need < maxb ? need : maxb;
+    s->persistent_grid = grid;
+    if (s->n > 0) {
+      hipError_t e = launch_jacobi_persistent(b, s->C, P, src, grid > 0 ? grid : 1, P.block, s->stream);
+      if (e != hipSuccess)
+        return fail(WSPH_E_UNSUPPORTED, "cooperative launch failed: %s", hipGetErrorString(e));
+    }
+    s->final_dp = 0;
+  } else {
+    // one launch per kernel and a host convergence test each iteration,
+    // as the reference's while loop (IISPH.py:1200-1296)
+    const int max_it = P.force_iters > 0 ? P.force_iters : P.max_iters;
+    int k = 0;
+    double e = 0.0;
+    int cont = 1;
+    for (; cont; ++k) {
+      launch_j1(b, s->C, src, P.block, s->stream);
+      launch_j2(b, s->C, src, k, P.block, s->stream);
+      HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
+      HIP_TRY(hipStreamSynchronize(s->stream));
+      cont = jacobi_continue(k, s->st_host->res_sum[k], s->st_host->res_cnt[k], P.min_iters,
+                             max_it, P.force_iters, P.eta, &e);
+      if (cont) src ^= 1;
+    }
+    k_solve_result<<<1, 1, 0, s->stream>>>(s->st, k, e, e <= P.eta, k);
+    s->final_dp = src;
+  }
+  HIP_TRY(hipGetLastError());
+  s->solved = true;
+  (void)stats;
+  return WSPH_OK;
+}
+
+static int read_stats(wsph_solver *s, wsph_step_stats *stats) {
+  HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const DevState &d = *s->st_host;
+  if (stats) {
+    stats->iterations = d.iterations;
+    stats->converged = d.converged;
+    stats->ap_evals = d.ap_evals;
+    stats->max_ff_count = d.max_ff;
+    stats->max_fs_count = d.max_fs;
+    stats->status = d.status;
+    stats->residual = d.residual;
+    stats->sim_time = d.sim_time;
+    stats->v_max = (double)__builtin_bit_cast(float, d.vmax_bits);
+  }
+  if (d.status & ST_BARRIER) return fail(WSPH_E_BARRIER, "persistent kernel grid barrier timed out");
+  if (d.status & (ST_FF_OVERFLOW | ST_FS_OVERFLOW))
+    return fail(WSPH_E_CAPACITY, "neighbour list capacity exceeded (fluid %d/%d, boundary %d/%d)",
+                d.max_ff, s->P.max_ff, d.max_fs, s->P.max_fs);
+  if (d.status & ST_NAN) return fail(WSPH_E_NAN, "non-finite particle state after the step");
+  return WSPH_OK;
+}
+
+int wsph_integrate(wsph_solver *s, wsph_step_stats *stats) {
+  if (!s || !s->solved) return fail(WSPH_E_ARG, "run wsph_pressure_solve first");
+  rec(s, 3);
+  const Bufs b = make_bufs(s);
+  if (stats) {
+    // dt used by this step, read before the clock update
+    HIP_TRY(hipMemcpyAsync(&s->st_host->dt, &s->st->dt, sizeof(double), hipMemcpyDeviceToHost,
+                           s->stream));
+  }
+  launch_integrate(b, s->C, s->final_dp, s->P.block, s->stream);
+  launch_clock(b, s->P, s->stream);
+  k_accumulate_totals<<<1, 1, 0, s->stream>>>(s->st, s->totals);
+  rec(s, 4);
+  HIP_TRY(hipGetLastError());
+  s->searched = s->advected = s->solved = false;
+  if (s->timing) s->t_recorded++;
+  if (stats) {
+    double dt_used = 0.0;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dt_used = s->st_host->dt;
+    int rc = read_stats(s, stats);
+    stats->dt = dt_used;
+    return rc;
+  }
+  return WSPH_OK;
+}
+
+int wsph_step(wsph_solver *s, wsph_step_stats *stats) {
+  int rc;
+  if ((rc = wsph_neighbor_search(s))) return rc;
+  if ((rc = wsph_predict_advection(s))) return rc;
+  if ((rc = wsph_pressure_solve(s, stats))) return rc;
+  return wsph_integrate(s, stats);
+}
+
+int wsph_get_field(wsph_solver *s, int field, void *dst) {
+  if (!s || !dst) return fail(WSPH_E_ARG, "null argument");
+  const int n = s->n;
+  const int *orig = s->orig[s->cur];
+  const int c = s->cur;
+  switch (field) {
+    case WSPH_F_X: launch_scatter(s->pos[c], 4, 3, 0, orig, n, dst, s->stream); break;
+    case WSPH_F_V: launch_scatter(s->vel[c], 4, 3, 0, orig, n, dst, s->stream); break;
+    case WSPH_F_P: launch_scatter(s->pprev[c], 1, 1, 0, orig, n, dst, s->stream); break;
+    case WSPH_F_RHO: launch_scatter(s->pos[c], 4, 1, 3, orig, n, dst, s->stream); break;
+    case WSPH_F_RHO_ADV: launch_scatter(s->aux, 4, 1, 0, orig, n, dst, s->stream); break;
+    case WSPH_F_V_ADV: launch_scatter(s->vadv, 4, 3, 0, orig, n, dst, s->stream); break;
+    case WSPH_F_D_II: launch_scatter(s->dp[s->final_dp], 4, 3, 0, orig, n, dst, s->stream); break;
+    case WSPH_F_A_II: launch_scatter(s->aux, 4, 1, 1, orig, n, dst, s->stream); break;
+    case WSPH_F_AP: launch_scatter(s->ap, 1, 1, 0, orig, n, dst, s->stream); break;
+    case WSPH_F_S: launch_scatter(s->sap, 4, 3, 0, orig, n, dst, s->stream); break;
+    case WSPH_F_FF_COUNT: launch_scatter(s->cnt_ff, 1, 1, 0, orig, n, dst, s->stream); break;
+    case WSPH_F_FS_COUNT: launch_scatter(s->cnt_fs, 1, 1, 0, orig, n, dst, s->stream); break;
+    case WSPH_F_PSI: launch_scatter(s->bpos, 4, 1, 3, s->borig, s->nb, dst, s->stream); break;
+    case WSPH_F_CELL_KEY:
+      if (n > 0) {
+        k_keys4<<<nblk(n, 256), 256, 0, s->stream>>>(s->pos[c], n, s->C.cell_inv, s->P.grid_dims[0],
+                                                     s->P.grid_dims[1], s->P.grid_dims[2], s->tmp);
+        launch_scatter(s->tmp, 1, 1, 0, orig, n, dst, s->stream);
+      }
+      break;
+    default: return fail(WSPH_E_ARG, "unknown field %d", field);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return WSPH_OK;
+}
+
+int wsph_get_neighbors(wsph_solver *s, int which, int *cnt, int *idx, int cap) {
+  if (!s || !cnt || !idx || cap < 1) return fail(WSPH_E_ARG, "bad arguments");
+  if (which != 0 && which != 1) return fail(WSPH_E_ARG, "which must be 0 or 1");
+  const int n = s->n;
+  if (n > 0) {
+    if (which == 0)
+      k_export_lists<<<nblk(n, 256), 256, 0, s->stream>>>(s->nbr_ff, s->cnt_ff, n, s->orig[s->cur],
+                                                          s->orig[s->cur], s->P.max_ff, cap, cnt, idx);
+    else
+      k_export_lists<<<nblk(n, 256), 256, 0, s->stream>>>(s->nbr_fs, s->cnt_fs, n, s->orig[s->cur],
+                                                          s->borig, s->P.max_fs, cap, cnt, idx);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return WSPH_OK;
+}
+
+int wsph_get_time(wsph_solver *s, double *sim_time, double *dt) {
+  if (!s) return fail(WSPH_E_ARG, "null solver");
+  HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (sim_time) *sim_time = s->st_host->sim_time;
+  if (dt) *dt = s->st_host->dt;
+  return WSPH_OK;
+}
+
+int wsph_get_counters(wsph_solver *s, long long *out, int nout) {
+  if (!s || !out || nout < 1) return fail(WSPH_E_ARG, "bad arguments");
+  long long h[8];
+  HIP_TRY(hipMemcpyAsync(h, s->totals, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < nout && k < 8; ++k) out[k] = h[k];
+  return WSPH_OK;
+}
+
+int wsph_hash_keys(const float *x, int n, float cell_width, const int dims[3], int *keys) {
+  if (n < 0 || (n > 0 && (!x || !keys)) || !dims || !(cell_width > 0))
+    return fail(WSPH_E_ARG, "bad arguments");
+  int rc;
+  if ((rc = check_dims(dims, "hash"))) return rc;
+  linear_keys(x, n, 1.0f / cell_width, dims, keys, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  return WSPH_OK;
+}
+
+int wsph_set_timing(wsph_solver *s, int enable) {
+  if (!s) return fail(WSPH_E_ARG, "null solver");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->timing = enable != 0;
+  s->t_recorded = s->t_processed = 0;
+  for (double &v : s->t_sum) v = 0.0;
+  return WSPH_OK;
+}
+
+int wsph_last_timings(wsph_solver *s, double *out, int nout) {
+  if (!s || !out) return fail(WSPH_E_ARG, "null argument");
+  while (s->t_processed < s->t_recorded) timing_fold_one(s);
+  double v[8] = {s->t_sum[0], s->t_sum[1], s->t_sum[2], s->t_sum[3], s->t_sum[4],
+                 (double)s->persistent_grid, (double)s->t_processed, 0.0};
+  for (int k = 0; k < nout && k < 8; ++k) out[k] = v[k];
+  return WSPH_OK;
+}
+
+int wsph_sync(wsph_solver *s) {
+  if (!s) return fail(WSPH_E_ARG, "null solver");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return WSPH_OK;
+}
+
+}  // extern "C"

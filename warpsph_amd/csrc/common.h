@@ -1,0 +1,168 @@
+// common.h -- device-side helpers shared by the libwsph kernels (gfx950).
+//
+// All kernels are compiled with -ffp-contract=off: every float32 multiply and
+// add rounds separately, in the operation order of the reference kernels
+// (wp_sph/solver/IISPH.py, sph_funcs.py), so the results reproduce the CPU
+// restatement bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/wsph.h"
+
+namespace wsph {
+
+constexpr int kMaxIters = 128;      // upper bound of max_iters / force_iters
+constexpr int kMaxLdsTable = 16384; // floats per table kept in LDS (64 KiB)
+
+// status bits accumulated on the device during a step
+enum : int {
+  ST_FF_OVERFLOW = 1,
+  ST_FS_OVERFLOW = 2,
+  ST_NAN = 4,
+  ST_BARRIER = 8,
+};
+
+// Constants of one solver, passed by value to every kernel (the float32
+// literals the reference's kernels see, wp_sph/main.py:25-61).
+struct SphConst {
+  float h;        // SMOOTHING_LENGTH
+  float cell_inv; // 1.0f / h, hash grid cell width inverse
+  float w_scale;  // float(W_SIZE - 1) * h_inv   (sph_funcs.py:98)
+  float g_scale;  // float(GRAD_W_SIZE - 1) * h_inv (sph_funcs.py:113)
+  int nw, ng;     // table sizes
+  float mass, rho0, omega, inv_small, g_y, vis_mu, xsph;
+  int fdims[3], bdims[3]; // hash dims of the fluid and boundary grids
+  int layout;             // WSPH_LAYOUT_*
+};
+
+// Device-resident step state: clock, residual accumulators, counters.
+struct DevState {
+  double dt;                  // dt of the coming step
+  double sim_time;            // simulated time
+  double residual;            // mean density error after the last iteration
+  unsigned vmax_bits;         // max |v| (float bits, non-negative)
+  int iterations;             // Jacobi iterations of the last solve
+  int ap_evals;
+  int converged;
+  int max_ff, max_fs;         // largest neighbour counts seen
+  int status;                 // ST_* bits
+  int done;                   // Jacobi loop finished flag
+  unsigned bar_count, bar_gen;  // grid barrier
+  unsigned long long res_sum[kMaxIters];
+  unsigned res_cnt[kMaxIters];
+};
+
+// ---- warp.HashGrid cell arithmetic (restated; used at IISPH.py:771-780) ----
+__device__ __forceinline__ int hash_axis(int c, int dim) {
+  c += 1 << 20;
+  c = c < 0 ? 0 : c;
+  return c % dim;
+}
+
+__host__ __device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  auto sp = [](uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+  };
+  return sp(x) | (sp(y) << 1) | (sp(z) << 2);
+}
+
+// Slot of hashed cell (hx, hy, hz) in the cell tables.
+__device__ __forceinline__ int cell_slot(int hx, int hy, int hz, const int *dims, int layout) {
+  if (layout == WSPH_LAYOUT_MORTON) return (int)morton3(hx, hy, hz);
+  return hz * (dims[0] * dims[1]) + hy * dims[0] + hx;
+}
+
+// Reference linear key cz*dx*dy + cy*dx + cx (hash_grid_index).
+__device__ __forceinline__ int linear_key(float x, float y, float z, float inv, const int *dims) {
+  const int hx = hash_axis((int)(x * inv), dims[0]);
+  const int hy = hash_axis((int)(y * inv), dims[1]);
+  const int hz = hash_axis((int)(z * inv), dims[2]);
+  return hz * (dims[0] * dims[1]) + hy * dims[0] + hx;
+}
+
+// Query range of hash_grid_query on one axis: [trunc((p-r)/w),
+// min(trunc((p+r)/w), start + dim - 1)].
+struct Range3 {
+  int x0, x1, y0, y1, z0, z1;
+};
+__device__ __forceinline__ Range3 query_range(float px, float py, float pz, float r, float inv,
+                                              const int *dims) {
+  Range3 q;
+  q.x0 = (int)((px - r) * inv);
+  q.y0 = (int)((py - r) * inv);
+  q.z0 = (int)((pz - r) * inv);
+  q.x1 = min((int)((px + r) * inv), q.x0 + dims[0] - 1);
+  q.y1 = min((int)((py + r) * inv), q.y0 + dims[1] - 1);
+  q.z1 = min((int)((pz + r) * inv), q.z0 + dims[2] - 1);
+  return q;
+}
+
+// ---- kernel tables: spline_W / grad_spline_W (sph_funcs.py:92-118) --------
+__device__ __forceinline__ float spline_w(const float *T, int n, float scale, float h, float r) {
+  if (r >= h) return 0.0f;
+  const float idx = r * scale;
+  int lo = (int)idx;
+  lo = lo < n - 2 ? lo : n - 2;
+  const float a = idx - (float)lo;
+  return (1.0f - a) * T[lo] + a * T[lo + 1];
+}
+
+// grad of W at x_j - x_i = r, i.e. the table value (-W'/|r|) times r.
+__device__ __forceinline__ float3 grad_w(const float *G, int n, float scale, float h, float3 r) {
+  const float len = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+  if (len >= h || len < 1e-10f) return make_float3(0.0f, 0.0f, 0.0f);
+  const float idx = len * scale;
+  int lo = (int)idx;
+  lo = lo < n - 2 ? lo : n - 2;
+  const float a = idx - (float)lo;
+  const float s = (1.0f - a) * G[lo] + a * G[lo + 1];
+  return make_float3(s * r.x, s * r.y, s * r.z);
+}
+
+__device__ __forceinline__ float dist3(float ax, float ay, float az, float bx, float by, float bz) {
+  const float dx = ax - bx, dy = ay - by, dz = az - bz;
+  return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Copy a table into LDS (when small enough) or use it from global memory.
+__device__ __forceinline__ const float *stage_table(float *lds, const float *g, int n, bool use_lds) {
+  if (!use_lds) return g;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) lds[k] = g[k];
+  __syncthreads();
+  return lds;
+}
+
+// Fixed-point residual term of one particle (WSPH_RESIDUAL_* in wsph.h).
+__device__ __forceinline__ unsigned long long residual_fixed(float term) {
+  if (term > WSPH_RESIDUAL_TERM_CLAMP) term = WSPH_RESIDUAL_TERM_CLAMP;
+  return (unsigned long long)((double)term * WSPH_RESIDUAL_FIXED_SCALE);
+}
+
+// wave64 reductions
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (unsigned)__shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+}  // namespace wsph

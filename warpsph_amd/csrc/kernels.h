@@ -1,0 +1,95 @@
+// kernels.h -- internal interface between the C-ABI layer and the kernels.
+#pragma once
+
+#include "common.h"
+
+namespace wsph {
+
+// ---- hash grid build (grid.hip) ----------------------------------------
+struct GridArgs {
+  int n;
+  long ncell;
+  float inv;
+  int dims[3];
+  int layout;
+  const float4 *pos;   // current order
+  const int *orig;     // original index of each current slot
+  int *slot, *rank, *tmp, *perm;
+  unsigned *count, *start;
+  unsigned **scan_bufs;
+};
+
+int scan_levels_needed(long n, long *sizes, int max_levels);
+void exclusive_scan(const unsigned *in, unsigned *out, long n, unsigned **level_bufs,
+                    hipStream_t st);
+void sort_into_cells(const GridArgs &g, hipStream_t st);
+void gather_fluid(const int *perm, int n, const float4 *pos, const float4 *vel, const float *pprev,
+                  const int *orig, float4 *pos_o, float4 *vel_o, float *pprev_o, int *orig_o,
+                  hipStream_t st);
+void fill_u32(unsigned *p, unsigned v, long n, hipStream_t st);
+void linear_keys(const float *x, int n, float inv, const int *dims, int *keys, hipStream_t st);
+
+// ---- SPH passes (sph.hip) ------------------------------------------------
+// Every per-particle array is in cell-sorted order.
+struct Bufs {
+  int n, nb;
+  // fluid
+  float4 *pos;      // x, y, z, rho
+  float4 *vel;      // v
+  float4 *vadv;     // v_adv
+  float4 *dp[2];    // d_ii.xyz, p   (double buffer of the Jacobi iterate)
+  float4 *sap;      // s = sum_j d_ij p_j (xyz), first part of A p (w)
+  float4 *aux;      // rho_adv, a_ii
+  float *ap;        // A p
+  float *pprev;     // pressure carried to the next step
+  const float *pstart;  // caller-supplied start pressure, sorted order (or null)
+  // fluid grid
+  const unsigned *fstart, *fcount;
+  // neighbour lists, column-major: list[k * n + i]
+  int *nbr_ff, *cnt_ff, *nbr_fs, *cnt_fs;
+  int cap_ff, cap_fs;
+  // boundary (sorted by its own grid)
+  float4 *bpos;     // x, y, z, psi
+  float4 *bvel;
+  const unsigned *bstart, *bcount;
+  // tables
+  const float *wt, *gt;
+  bool lds_tables;
+  DevState *st;
+};
+
+void launch_boundary_psi(const Bufs &b, const SphConst &c, hipStream_t s);
+void launch_lists(const Bufs &b, const SphConst &c, int block, hipStream_t s);
+void launch_density(const Bufs &b, const SphConst &c, bool fuse_vadv, int block, hipStream_t s);
+void launch_vadv(const Bufs &b, const SphConst &c, int block, hipStream_t s);
+// start: WSPH_WARM_* ; writes dp[0] = (d_ii, p1) for a zero start, else (d_ii, p0)
+void launch_assemble(const Bufs &b, const SphConst &c, int start_mode, int block, hipStream_t s);
+// One A p evaluation on dp[src]; writes dp[1-src] = next iterate. slot >= 0
+// accumulates the residual of iteration `slot`.
+void launch_j1(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s);
+void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, hipStream_t s);
+void launch_integrate(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s);
+void launch_clock(const Bufs &b, const wsph_params &P, hipStream_t s);
+void launch_reset_state(const Bufs &b, hipStream_t s);
+// Persistent cooperative Jacobi loop; returns a hip error code.
+hipError_t launch_jacobi_persistent(const Bufs &b, const SphConst &c, const wsph_params &P,
+                                    int src, int grid, int block, hipStream_t s);
+int persistent_max_blocks(const Bufs &b, const SphConst &c, int block, int *blocks_per_cu);
+// Field scatter back to original order
+void launch_scatter(const void *src, int stride_floats, int ncomp, int src_off, const int *orig,
+                    int n, void *dst, hipStream_t s);
+
+// Jacobi stop rule shared by the host loop and the persistent kernel
+// (IISPH.py:1200-1207): returns 1 to continue after iteration k.
+__host__ __device__ inline int jacobi_continue(int k, unsigned long long sum, unsigned cnt,
+                                               int min_it, int max_it, int force_it, double eta,
+                                               double *e_out) {
+  const double e = cnt == 0 ? 0.0 : ((double)sum / WSPH_RESIDUAL_FIXED_SCALE) / (double)cnt;
+  *e_out = e;
+  if (force_it > 0) return k + 1 < force_it;
+  if (k + 1 >= max_it) return 0;
+  if (k + 1 < min_it) return 1;
+  return e > eta;
+}
+
+}  // namespace wsph

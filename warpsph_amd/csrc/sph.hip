@@ -1,0 +1,646 @@
+// sph.hip -- IISPH neighbour passes on gfx950.
+//
+// One thread per particle (cell-sorted order). Every sum runs over the
+// particle's neighbours in the reference's enumeration order and every
+// expression keeps the reference's operation order (IISPH.py:23-699), so the
+// float32 results equal those of the CPU restatement exactly. Neighbour
+// lists are column-major (list[k * n + i]) so that lane i of a wave reads
+// slot k of 64 consecutive particles in one coalesced access. The kernel
+// tables sit in LDS when they fit (sizes up to kMaxLdsTable floats).
+#include "kernels.h"
+
+namespace wsph {
+
+extern __shared__ float g_smem[];
+
+struct Tabs {
+  const float *w, *g;
+};
+
+template <bool kLds>
+__device__ __forceinline__ Tabs stage_tabs(const Bufs &b, const SphConst &c) {
+  if constexpr (kLds) {
+    for (int k = threadIdx.x; k < c.nw; k += blockDim.x) g_smem[k] = b.wt[k];
+    for (int k = threadIdx.x; k < c.ng; k += blockDim.x) g_smem[c.nw + k] = b.gt[k];
+    __syncthreads();
+    return Tabs{g_smem, g_smem + c.nw};
+  } else {
+    return Tabs{b.wt, b.gt};
+  }
+}
+
+#define W_OF(T, r) spline_w((T).w, c.nw, c.w_scale, c.h, (r))
+#define GW_OF(T, r) grad_w((T).g, c.ng, c.g_scale, c.h, (r))
+
+__device__ __forceinline__ float3 sub3(float4 a, float4 b) {
+  return make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+
+// update_p, IISPH.py:521-539
+__device__ __forceinline__ float update_p(float p, float rho_adv, float ap, float a_ii,
+                                          const SphConst &c) {
+  float a = a_ii;
+  if (a > -c.inv_small && a < c.inv_small) a = (a < 0.0f ? -1.0f : 1.0f) * c.inv_small;
+  const float pi = p + (c.rho0 - rho_adv - ap) * c.omega / a;
+  return pi > 0.0f ? pi : 0.0f;
+}
+
+// ---- compute_boundary_density, IISPH.py:23-47 --------------------------
+template <bool kLds>
+__global__ void k_boundary_psi(Bufs b, SphConst c) {
+  const Tabs T = stage_tabs<kLds>(b, c);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.nb) return;
+  const float4 p = b.bpos[i];
+  float rho = W_OF(T, 0.0f);
+  const Range3 q = query_range(p.x, p.y, p.z, c.h, c.cell_inv, c.bdims);
+  for (int z = q.z0; z <= q.z1; ++z)
+    for (int y = q.y0; y <= q.y1; ++y)
+      for (int x = q.x0; x <= q.x1; ++x) {
+        const int sl = cell_slot(hash_axis(x, c.bdims[0]), hash_axis(y, c.bdims[1]),
+                                 hash_axis(z, c.bdims[2]), c.bdims, c.layout);
+        const int s = (int)b.bstart[sl], e = s + (int)b.bcount[sl];
+        for (int u = s; u < e; ++u) {
+          const float4 o = b.bpos[u];
+          const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
+          if (d < c.h && u != i) rho += W_OF(T, d);
+        }
+      }
+  reinterpret_cast<float *>(&b.bpos[i])[3] = c.rho0 / rho;
+}
+
+// ---- count_/store_same_neighbor (IISPH.py:52-104) and
+//      count_/store_diff_neighbor (IISPH.py:107-163) --------------------------
+__global__ void k_lists(Bufs b, SphConst c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int nf = 0, ns = 0;
+  if (i < b.n) {
+    const float4 p = b.pos[i];
+    const int n = b.n;
+    Range3 q = query_range(p.x, p.y, p.z, c.h, c.cell_inv, c.fdims);
+    for (int z = q.z0; z <= q.z1; ++z)
+      for (int y = q.y0; y <= q.y1; ++y)
+        for (int x = q.x0; x <= q.x1; ++x) {
+          const int sl = cell_slot(hash_axis(x, c.fdims[0]), hash_axis(y, c.fdims[1]),
+                                   hash_axis(z, c.fdims[2]), c.fdims, c.layout);
+          const int s = (int)b.fstart[sl], e = s + (int)b.fcount[sl];
+          for (int j = s; j < e; ++j) {
+            if (j == i) continue;
+            const float4 o = b.pos[j];
+            if (dist3(p.x, p.y, p.z, o.x, o.y, o.z) < c.h) {
+              if (nf < b.cap_ff) b.nbr_ff[(long)nf * n + i] = j;
+              ++nf;
+            }
+          }
+        }
+    q = query_range(p.x, p.y, p.z, c.h, c.cell_inv, c.bdims);
+    if (b.nb > 0)
+      for (int z = q.z0; z <= q.z1; ++z)
+        for (int y = q.y0; y <= q.y1; ++y)
+          for (int x = q.x0; x <= q.x1; ++x) {
+            const int sl = cell_slot(hash_axis(x, c.bdims[0]), hash_axis(y, c.bdims[1]),
+                                     hash_axis(z, c.bdims[2]), c.bdims, c.layout);
+            const int s = (int)b.bstart[sl], e = s + (int)b.bcount[sl];
+            for (int j = s; j < e; ++j) {
+              const float4 o = b.bpos[j];
+              if (dist3(p.x, p.y, p.z, o.x, o.y, o.z) < c.h) {
+                if (ns < b.cap_fs) b.nbr_fs[(long)ns * n + i] = j;
+                ++ns;
+              }
+            }
+          }
+    b.cnt_ff[i] = nf < b.cap_ff ? nf : b.cap_ff;
+    b.cnt_fs[i] = ns < b.cap_fs ? ns : b.cap_fs;
+  }
+  const int mf = wave_max_i32(nf), ms = wave_max_i32(ns);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&b.st->max_ff, mf);
+    atomicMax(&b.st->max_fs, ms);
+    if (mf > b.cap_ff) atomicOr(&b.st->status, (int)ST_FF_OVERFLOW);
+    if (ms > b.cap_fs) atomicOr(&b.st->status, (int)ST_FS_OVERFLOW);
+  }
+}
+
+// v_adv of predict_v_adv (IISPH.py:217-278) with the Laplacian term switched
+// off (VIS_MU = 0): v + ((0 * vis_mu) / rho + g) dt.
+__device__ __forceinline__ float4 vadv_plain(float4 v, float rho, float dt, const SphConst &c) {
+  const float gx = 0.0f, gy = c.g_y, gz = 0.0f;
+  const float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+  return make_float4(v.x + (lx * c.vis_mu / rho + gx) * dt, v.y + (ly * c.vis_mu / rho + gy) * dt,
+                     v.z + (lz * c.vis_mu / rho + gz) * dt, 0.0f);
+}
+
+// ---- compute_density, IISPH.py:177-214 -----------------------------------
+template <bool kLds>
+__global__ void k_density(Bufs b, SphConst c, int fuse_vadv) {
+  const Tabs T = stage_tabs<kLds>(b, c);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  const int n = b.n;
+  const float4 p = b.pos[i];
+  float t1 = 0.0f, t2 = 0.0f;
+  const int nf = b.cnt_ff[i], ns = b.cnt_fs[i];
+  for (int k = 0; k < nf; ++k) {
+    const float4 o = b.pos[b.nbr_ff[(long)k * n + i]];
+    t1 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z));
+  }
+  for (int k = 0; k < ns; ++k) {
+    const float4 o = b.bpos[b.nbr_fs[(long)k * n + i]];
+    t2 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z)) * o.w;
+  }
+  const float rho = t1 * c.mass + t2 + W_OF(T, 0.0f) * c.mass;
+  reinterpret_cast<float *>(&b.pos[i])[3] = rho;
+  if (fuse_vadv) b.vadv[i] = vadv_plain(b.vel[i], rho, (float)b.st->dt, c);
+}
+
+// ---- predict_v_adv, IISPH.py:217-278 (+ XSPH smoothing) ------------------
+template <bool kLds>
+__global__ void k_vadv(Bufs b, SphConst c) {
+  const Tabs T = stage_tabs<kLds>(b, c);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  const int n = b.n;
+  const float dt = (float)b.st->dt;
+  const float4 p = b.pos[i];
+  const float4 vi = b.vel[i];
+  const int nf = b.cnt_ff[i], ns = b.cnt_fs[i];
+  float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+  if (c.vis_mu != 0.0f) {
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
+    for (int k = 0; k < nf; ++k) {
+      const int j = b.nbr_ff[(long)k * n + i];
+      const float4 o = b.pos[j];
+      const float4 vj = b.vel[j];
+      const float3 xij = sub3(o, p);
+      const float3 gw = GW_OF(T, xij);
+      const float3 vd = make_float3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z);
+      const float dd = dot3(vd, xij);
+      const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
+      const float r2 = d * d;
+      ax = ax - gw.x * dd / o.w / r2;
+      ay = ay - gw.y * dd / o.w / r2;
+      az = az - gw.z * dd / o.w / r2;
+    }
+    for (int k = 0; k < ns; ++k) {
+      const float4 o = b.bpos[b.nbr_fs[(long)k * n + i]];
+      const float3 xib = sub3(o, p);
+      const float3 gw = GW_OF(T, xib);
+      const float dd = dot3(make_float3(vi.x, vi.y, vi.z), xib);
+      const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
+      const float r2 = d * d;
+      bx = bx - gw.x * o.w * dd / r2;
+      by = by - gw.y * o.w * dd / r2;
+      bz = bz - gw.z * o.w * dd / r2;
+    }
+    lx = 10.0f * (ax * c.mass + bx / c.rho0);
+    ly = 10.0f * (ay * c.mass + by / c.rho0);
+    lz = 10.0f * (az * c.mass + bz / c.rho0);
+  }
+  float vx = vi.x + (lx * c.vis_mu / p.w + 0.0f) * dt;
+  float vy = vi.y + (ly * c.vis_mu / p.w + c.g_y) * dt;
+  float vz = vi.z + (lz * c.vis_mu / p.w + 0.0f) * dt;
+  if (c.xsph != 0.0f) {
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (int k = 0; k < nf; ++k) {
+      const int j = b.nbr_ff[(long)k * n + i];
+      const float4 o = b.pos[j];
+      const float4 vj = b.vel[j];
+      const float coef = c.mass / o.w * W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z));
+      ax = ax + (vj.x - vi.x) * coef;
+      ay = ay + (vj.y - vi.y) * coef;
+      az = az + (vj.z - vi.z) * coef;
+    }
+    vx = vx + ax * c.xsph;
+    vy = vy + ay * c.xsph;
+    vz = vz + az * c.xsph;
+  }
+  b.vadv[i] = make_float4(vx, vy, vz, 0.0f);
+}
+
+// ---- predict_rho_adv (IISPH.py:281-329), compute_term_d (IISPH.py:332-382)
+//      and compute_term_a (IISPH.py:385-428) in one pass ---------------------
+template <bool kLds>
+__global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
+  const Tabs T = stage_tabs<kLds>(b, c);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  const int n = b.n;
+  const float dt = (float)b.st->dt;
+  const float dt2 = dt * dt;
+  const float t3 = c.mass * dt2;
+  const float4 p = b.pos[i];
+  const float4 va = b.vadv[i];
+  const float ri2 = p.w * p.w;
+  const int nf = b.cnt_ff[i], ns = b.cnt_fs[i];
+  float tr1 = 0.0f, tr2 = 0.0f;
+  float dx1 = 0.0f, dy1 = 0.0f, dz1 = 0.0f, dx2 = 0.0f, dy2 = 0.0f, dz2 = 0.0f;
+  for (int k = 0; k < nf; ++k) {
+    const int j = b.nbr_ff[(long)k * n + i];
+    const float4 o = b.pos[j];
+    const float4 vj = b.vadv[j];
+    const float3 gw = GW_OF(T, sub3(o, p));
+    tr1 += dot3(make_float3(va.x - vj.x, va.y - vj.y, va.z - vj.z), gw);
+    dx1 = dx1 - gw.x;
+    dy1 = dy1 - gw.y;
+    dz1 = dz1 - gw.z;
+  }
+  for (int k = 0; k < ns; ++k) {
+    const int j = b.nbr_fs[(long)k * n + i];
+    const float4 o = b.bpos[j];
+    const float4 vb = b.bvel[j];
+    const float3 gw = GW_OF(T, sub3(o, p));
+    tr2 += dot3(make_float3(va.x - vb.x, va.y - vb.y, va.z - vb.z), gw) * o.w;
+    dx2 = dx2 - gw.x * o.w;
+    dy2 = dy2 - gw.y * o.w;
+    dz2 = dz2 - gw.z * o.w;
+  }
+  const float rho_adv = p.w + (tr1 * c.mass + tr2) * dt;
+  const float dix = (dx1 * c.mass + dx2) * dt2 / ri2;
+  const float diy = (dy1 * c.mass + dy2) * dt2 / ri2;
+  const float diz = (dz1 * c.mass + dz2) * dt2 / ri2;
+  float ta1 = 0.0f, ta2 = 0.0f;
+  for (int k = 0; k < nf; ++k) {
+    const float4 o = b.pos[b.nbr_ff[(long)k * n + i]];
+    const float3 gw = GW_OF(T, sub3(o, p));
+    const float3 dd = make_float3(dix - gw.x * t3 / ri2, diy - gw.y * t3 / ri2,
+                                  diz - gw.z * t3 / ri2);
+    ta1 += dot3(dd, gw);
+  }
+  for (int k = 0; k < ns; ++k) {
+    const float4 o = b.bpos[b.nbr_fs[(long)k * n + i]];
+    const float3 gw = GW_OF(T, sub3(o, p));
+    ta2 += dot3(make_float3(dix, diy, diz), gw) * o.w;
+  }
+  const float a_ii = ta1 * c.mass + ta2;
+  b.aux[i] = make_float4(rho_adv, a_ii, 0.0f, 0.0f);
+  float pv;
+  if (start_mode == WSPH_WARM_ZERO)
+    pv = update_p(0.0f, rho_adv, 0.0f, a_ii, c);  // A 0 = 0: the first Jacobi update
+  else if (start_mode == WSPH_WARM_PREVIOUS)
+    pv = b.pprev[i];
+  else
+    pv = b.pstart[i];
+  b.dp[0][i] = make_float4(dix, diy, diz, pv);
+  b.sap[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  b.ap[i] = 0.0f;
+}
+
+// ---- compute_term_Ap_1, IISPH.py:431-466 ---------------------------------
+template <typename TT>
+__device__ __forceinline__ void j1_particle(const Bufs &b, const SphConst &c, const TT &T, int i,
+                                            int src, float t3) {
+  const int n = b.n;
+  const float4 p = b.pos[i];
+  const float4 *__restrict__ dp = b.dp[src];
+  const float pi = dp[i].w;
+  const float ri2 = p.w * p.w;
+  const int nf = b.cnt_ff[i];
+  float t1 = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  for (int k = 0; k < nf; ++k) {
+    const int j = b.nbr_ff[(long)k * n + i];
+    const float4 o = b.pos[j];
+    const float4 dj = dp[j];
+    const float3 gw = GW_OF(T, sub3(o, p));
+    const float3 term = make_float3(gw.x * t3 / ri2 * pi - dj.x * dj.w,
+                                    gw.y * t3 / ri2 * pi - dj.y * dj.w,
+                                    gw.z * t3 / ri2 * pi - dj.z * dj.w);
+    t1 += dot3(term, gw);
+    const float rj2 = o.w * o.w;
+    sx = sx + -gw.x * t3 / rj2 * dj.w;
+    sy = sy + -gw.y * t3 / rj2 * dj.w;
+    sz = sz + -gw.z * t3 / rj2 * dj.w;
+  }
+  b.sap[i] = make_float4(sx, sy, sz, t1 * c.mass);
+}
+
+// ---- compute_term_Ap_2 (IISPH.py:469-518), update_rho_error (IISPH.py:542-568)
+//      and the next update_p (IISPH.py:521-539) ----------------------------
+template <typename TT>
+__device__ __forceinline__ void j2_particle(const Bufs &b, const SphConst &c, const TT &T, int i,
+                                            int src, unsigned long long &rsum, unsigned &rcnt) {
+  const int n = b.n;
+  const float4 p = b.pos[i];
+  const float4 si = b.sap[i];
+  const int nf = b.cnt_ff[i], ns = b.cnt_fs[i];
+  float t1 = 0.0f, t2 = 0.0f;
+  for (int k = 0; k < nf; ++k) {
+    const int j = b.nbr_ff[(long)k * n + i];
+    const float4 o = b.pos[j];
+    const float4 sj = b.sap[j];
+    const float3 gw = GW_OF(T, sub3(o, p));
+    t1 += dot3(make_float3(si.x - sj.x, si.y - sj.y, si.z - sj.z), gw);
+  }
+  for (int k = 0; k < ns; ++k) {
+    const float4 o = b.bpos[b.nbr_fs[(long)k * n + i]];
+    const float3 gw = GW_OF(T, sub3(o, p));
+    t2 += dot3(make_float3(si.x, si.y, si.z), gw) * o.w;
+  }
+  const float4 di = b.dp[src][i];
+  const float4 au = b.aux[i];
+  const float ap = si.w + (t1 * c.mass + t2 + au.y * di.w);
+  b.ap[i] = ap;
+  const float ur = au.x + ap;
+  if (ur > c.rho0) {
+    rsum += residual_fixed(fabsf(ur / c.rho0 - 1.0f));
+    rcnt += 1u;
+  }
+  b.dp[src ^ 1][i] = make_float4(di.x, di.y, di.z, update_p(di.w, au.x, ap, au.y, c));
+}
+
+__device__ __forceinline__ void block_residual_add(DevState *st, int slot,
+                                                   unsigned long long rsum, unsigned rcnt) {
+  __shared__ unsigned long long s_sum[16];
+  __shared__ unsigned s_cnt[16];
+  rsum = wave_sum_u64(rsum);
+  rcnt = wave_sum_u32(rcnt);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_sum[wid] = rsum;
+    s_cnt[wid] = rcnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long S = 0;
+    unsigned Cn = 0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
+      S += s_sum[w];
+      Cn += s_cnt[w];
+    }
+    if (Cn) {
+      atomicAdd(&st->res_sum[slot], S);
+      atomicAdd(&st->res_cnt[slot], Cn);
+    }
+  }
+  __syncthreads();
+}
+
+template <bool kLds>
+__global__ void k_j1(Bufs b, SphConst c, int src) {
+  const Tabs T = stage_tabs<kLds>(b, c);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const float dt = (float)b.st->dt;
+  const float t3 = c.mass * (dt * dt);
+  if (i < b.n) j1_particle(b, c, T, i, src, t3);
+}
+
+template <bool kLds>
+__global__ void k_j2(Bufs b, SphConst c, int src, int slot) {
+  const Tabs T = stage_tabs<kLds>(b, c);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long rs = 0;
+  unsigned rc = 0;
+  if (i < b.n) j2_particle(b, c, T, i, src, rs, rc);
+  if (slot >= 0) block_residual_add(b.st, slot, rs, rc);
+}
+
+// ---- kick (IISPH.py:674-688) and drift (IISPH.py:691-699) ----------------
+__global__ void k_integrate(Bufs b, SphConst c, int src) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned vbits = 0;
+  int bad = 0;
+  if (i < b.n) {
+    const double dtd = b.st->dt;
+    const float dt = (float)dtd;
+    const float inv_dt = (float)(1.0 / dtd);
+    const float4 di = b.dp[src][i];
+    const float4 s = b.sap[i];
+    const float4 va = b.vadv[i];
+    const float vx = va.x + inv_dt * (di.x * di.w + s.x);
+    const float vy = va.y + inv_dt * (di.y * di.w + s.y);
+    const float vz = va.z + inv_dt * (di.z * di.w + s.z);
+    const float l = sqrtf(vx * vx + vy * vy + vz * vz);
+    vbits = __float_as_uint(l);
+    const float4 p = b.pos[i];
+    const float x = p.x + vx * dt, y = p.y + vy * dt, z = p.z + vz * dt;
+    b.pos[i] = make_float4(x, y, z, p.w);
+    b.vel[i] = make_float4(vx, vy, vz, 0.0f);
+    b.pprev[i] = di.w;
+    bad = !(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(l));
+    if (bad) vbits = 0;
+  }
+  vbits = wave_max_u32(vbits);
+  bad = wave_max_i32(bad);
+  if ((threadIdx.x & 63) == 0) {
+    if (vbits) atomicMax(&b.st->vmax_bits, vbits);
+    if (bad) atomicOr(&b.st->status, (int)ST_NAN);
+  }
+}
+
+// sim_time += dt; CFL dt update (IISPH.py:1367-1371)
+__global__ void k_clock(DevState *st, int dt_mode, double cfl_d, double inv_small, double dt_max) {
+  const double dt = st->dt;
+  st->sim_time = st->sim_time + dt;
+  if (dt_mode == WSPH_DT_CFL) {
+    const double vmax = (double)__uint_as_float(st->vmax_bits);
+    const double m = vmax > inv_small ? vmax : inv_small;
+    const double nd = cfl_d / m;
+    st->dt = nd < dt_max ? nd : dt_max;
+  }
+}
+
+__global__ void k_reset_state(DevState *st) {
+  const int t = threadIdx.x;
+  for (int k = t; k < kMaxIters; k += blockDim.x) {
+    st->res_sum[k] = 0ull;
+    st->res_cnt[k] = 0u;
+  }
+  if (t == 0) {
+    st->vmax_bits = 0u;
+    st->iterations = 0;
+    st->ap_evals = 0;
+    st->converged = 0;
+    st->max_ff = 0;
+    st->max_fs = 0;
+    st->status = 0;
+    st->done = 0;
+    st->residual = 0.0;
+    st->bar_count = 0u;
+    st->bar_gen = 0u;
+  }
+}
+
+// ---- persistent cooperative Jacobi loop -----------------------------------
+// Grid-wide barrier with a bounded wait: a wait that exceeds the spin limit
+// raises ST_BARRIER and every block leaves the loop, so the kernel always
+// ends.
+constexpr long kBarrierSpinLimit = 1L << 24;
+
+__device__ __forceinline__ bool grid_barrier(DevState *st) {
+  __shared__ int s_abort;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned nblk = gridDim.x;
+    const unsigned gen =
+        __hip_atomic_load(&st->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived =
+        __hip_atomic_fetch_add(&st->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    int abort = 0;
+    if (arrived == nblk - 1) {
+      __hip_atomic_store(&st->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&st->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long spins = 0;
+      while (__hip_atomic_load(&st->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > kBarrierSpinLimit) {
+          atomicOr(&st->status, (int)ST_BARRIER);
+          break;
+        }
+      }
+    }
+    abort = __hip_atomic_load(&st->status, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) &
+            ST_BARRIER;
+    s_abort = abort;
+  }
+  __syncthreads();
+  __threadfence();
+  return s_abort == 0;
+}
+
+template <bool kLds>
+__global__ void k_jacobi_persistent(Bufs b, SphConst c, int src, int min_it, int max_it,
+                                    int force_it, double eta) {
+  const Tabs T = stage_tabs<kLds>(b, c);
+  const float dt = (float)b.st->dt;
+  const float t3 = c.mass * (dt * dt);
+  const int stride = gridDim.x * blockDim.x;
+  const int first = blockIdx.x * blockDim.x + threadIdx.x;
+  int cur = src;
+  int k = 0;
+  double e = 0.0;
+  bool ok = true;
+  for (;; ++k) {
+    for (int i = first; i < b.n; i += stride) j1_particle(b, c, T, i, cur, t3);
+    if (!(ok = grid_barrier(b.st))) break;
+    unsigned long long rs = 0;
+    unsigned rc = 0;
+    for (int i = first; i < b.n; i += stride) j2_particle(b, c, T, i, cur, rs, rc);
+    block_residual_add(b.st, k, rs, rc);
+    if (!(ok = grid_barrier(b.st))) break;
+    const unsigned long long S =
+        __hip_atomic_load(&b.st->res_sum[k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned Cn =
+        __hip_atomic_load(&b.st->res_cnt[k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if (!jacobi_continue(k, S, Cn, min_it, max_it, force_it, eta, &e)) break;
+    cur ^= 1;
+  }
+  // leave the final iterate in dp[0]
+  if (ok && cur == 1)
+    for (int i = first; i < b.n; i += stride) b.dp[0][i] = b.dp[1][i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    b.st->iterations = k + 1;
+    b.st->ap_evals += k + 1;
+    b.st->residual = e;
+    b.st->converged = e <= eta;
+  }
+}
+
+// ---- scatter a field back to original order ------------------------------
+__global__ void k_scatter(const unsigned *__restrict__ src, int stride, int ncomp, int off,
+                          const int *__restrict__ orig, int n, unsigned *__restrict__ dst) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const long o = orig[f];
+  for (int k = 0; k < ncomp; ++k) dst[o * ncomp + k] = src[(long)f * stride + off + k];
+}
+
+// ---- launchers ---------------------------------------------------------------
+static inline unsigned nblocks(int n, int block) { return (unsigned)((n + block - 1) / block); }
+static inline size_t tab_bytes(const Bufs &b, const SphConst &c) {
+  return b.lds_tables ? sizeof(float) * (size_t)(c.nw + c.ng) : 0;
+}
+
+#define DISPATCH_LDS(kern, grid, block, shm, stream, ...)                    \
+  do {                                                                      \
+    if (b.lds_tables)                                                       \
+      kern<true><<<(grid), (block), (shm), (stream)>>>(__VA_ARGS__);        \
+    else                                                                    \
+      kern<false><<<(grid), (block), (shm), (stream)>>>(__VA_ARGS__);       \
+  } while (0)
+
+void launch_boundary_psi(const Bufs &b, const SphConst &c, hipStream_t s) {
+  if (b.nb == 0) return;
+  DISPATCH_LDS(k_boundary_psi, nblocks(b.nb, 256), 256, tab_bytes(b, c), s, b, c);
+}
+
+void launch_lists(const Bufs &b, const SphConst &c, int block, hipStream_t s) {
+  if (b.n == 0) return;
+  k_lists<<<nblocks(b.n, block), block, 0, s>>>(b, c);
+}
+
+void launch_density(const Bufs &b, const SphConst &c, bool fuse_vadv, int block, hipStream_t s) {
+  if (b.n == 0) return;
+  DISPATCH_LDS(k_density, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, (int)fuse_vadv);
+}
+
+void launch_vadv(const Bufs &b, const SphConst &c, int block, hipStream_t s) {
+  if (b.n == 0) return;
+  DISPATCH_LDS(k_vadv, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c);
+}
+
+void launch_assemble(const Bufs &b, const SphConst &c, int start_mode, int block, hipStream_t s) {
+  if (b.n == 0) return;
+  DISPATCH_LDS(k_assemble, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, start_mode);
+}
+
+void launch_j1(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s) {
+  if (b.n == 0) return;
+  DISPATCH_LDS(k_j1, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src);
+}
+
+void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, hipStream_t s) {
+  if (b.n == 0) return;
+  DISPATCH_LDS(k_j2, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src, slot);
+}
+
+void launch_integrate(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s) {
+  if (b.n == 0) return;
+  k_integrate<<<nblocks(b.n, block), block, 0, s>>>(b, c, src);
+}
+
+void launch_clock(const Bufs &b, const wsph_params &P, hipStream_t s) {
+  k_clock<<<1, 1, 0, s>>>(b.st, P.dt_mode, P.cfl * P.spacing, P.inv_small, P.dt_max);
+}
+
+void launch_reset_state(const Bufs &b, hipStream_t s) { k_reset_state<<<1, 128, 0, s>>>(b.st); }
+
+int persistent_max_blocks(const Bufs &b, const SphConst &c, int block, int *blocks_per_cu) {
+  int dev = 0, ncu = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return -1;
+  const size_t shm = tab_bytes(b, c);
+  hipError_t e;
+  if (b.lds_tables)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_jacobi_persistent<true>, block, shm);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_jacobi_persistent<false>, block, shm);
+  if (e != hipSuccess) return -1;
+  if (blocks_per_cu) *blocks_per_cu = per;
+  return per * ncu;
+}
+
+hipError_t launch_jacobi_persistent(const Bufs &b, const SphConst &c, const wsph_params &P,
+                                    int src, int grid, int block, hipStream_t s) {
+  Bufs bb = b;
+  SphConst cc = c;
+  int min_it = P.min_iters, max_it = P.max_iters, force_it = P.force_iters;
+  double eta = P.eta;
+  void *args[] = {&bb, &cc, &src, &min_it, &max_it, &force_it, &eta};
+  const unsigned shm = (unsigned)tab_bytes(b, c);
+  if (b.lds_tables)
+    return hipLaunchCooperativeKernel((const void *)k_jacobi_persistent<true>, dim3(grid),
+                                      dim3(block), args, shm, s);
+  return hipLaunchCooperativeKernel((const void *)k_jacobi_persistent<false>, dim3(grid),
+                                    dim3(block), args, shm, s);
+}
+
+void launch_scatter(const void *src, int stride_floats, int ncomp, int src_off, const int *orig,
+                    int n, void *dst, hipStream_t s) {
+  if (n == 0) return;
+  k_scatter<<<nblocks(n, 256), 256, 0, s>>>((const unsigned *)src, stride_floats, ncomp, src_off,
+                                            orig, n, (unsigned *)dst);
+}
+
+}  // namespace wsph

@@ -1,0 +1,146 @@
+"""Scene generators.
+
+``reference_box`` restates IISPH.init_particles (wp_sph/solver/IISPH.py:857-1000)
+for the reference's EMPTY scene: a lattice shell of wall particles built by a
+triple loop in x-major order (IISPH.py:872-912) and a fluid block from
+``np.arange`` + ``meshgrid(indexing="ij")`` (IISPH.py:941-949), both cast from
+float64 to float32 (IISPH.py:990-994). ``dam_break`` builds the lattice block
+used by the benchmark workloads in the same order and precision.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+from .config import SolverConfig
+
+
+@dataclass
+class Scene:
+    x: np.ndarray            # (n, 3) float32 fluid positions
+    xb: np.ndarray           # (nb, 3) float32 boundary positions
+    vb: np.ndarray           # (nb, 3) float32 boundary velocities
+    boundary_hide_n: int     # wall particles come first (IISPH.py:927)
+    bounds_lo: np.ndarray    # float64 lower corner of everything
+    bounds_hi: np.ndarray    # float64 upper corner of everything
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def nb(self) -> int:
+        return int(self.xb.shape[0])
+
+
+def box_shell(box: Sequence[float], spacing: float, layers: int) -> np.ndarray:
+    """Wall particles of ``layers`` lattice layers around the box interior
+    [0, int(B/d) - 1]^3, enumerated x outer, y, z inner (IISPH.py:872-912).
+    Returns float64 positions idx * spacing."""
+    ni = [int(b / spacing) for b in box]
+    rng = [np.arange(-layers, k + layers) for k in ni]
+    ix, iy, iz = np.meshgrid(*rng, indexing="ij")
+    ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+    outside = ((ix < 0) | (ix > ni[0] - 1) | (iy < 0) | (iy > ni[1] - 1)
+               | (iz < 0) | (iz > ni[2] - 1))
+    idx = np.stack([ix[outside], iy[outside], iz[outside]], axis=1)
+    return idx.astype(np.float64) * spacing
+
+
+def _bounds(*arrs: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    pts = [a for a in arrs if a.size]
+    lo = np.min([a.min(axis=0) for a in pts], axis=0).astype(np.float64)
+    hi = np.max([a.max(axis=0) for a in pts], axis=0).astype(np.float64)
+    return lo, hi
+
+
+def reference_box(cfg: SolverConfig) -> Scene:
+    """The reference's EMPTY scene (IISPH.py:763-766, 857-1000)."""
+    bw, bh, bl = cfg.box
+    min_point = cfg.fluid_min or (bw * 0.02, bh * 0.02, bl * 0.02)
+    max_point = cfg.fluid_max or (bw * 0.98, bh * 0.4, bl * 0.55)
+    for a in range(3):
+        if not (0 < min_point[a] < max_point[a] < cfg.box[a]):
+            raise ValueError("Invalid range for rectangle")  # IISPH.py:931-939
+    xb = box_shell(cfg.box, cfg.spacing, cfg.boundary_layers)
+    axes = [np.arange(min_point[a], max_point[a], cfg.spacing) for a in range(3)]
+    xx, yy, zz = np.meshgrid(*axes, indexing="ij")
+    x = np.stack([xx.ravel(), yy.ravel(), zz.ravel()]).T
+    x32 = x.astype(np.float32)
+    xb32 = xb.astype(np.float32)
+    lo, hi = _bounds(x32, xb32)
+    return Scene(x32, xb32, np.zeros_like(xb32), xb32.shape[0], lo, hi)
+
+
+def lattice_block(cells: Sequence[int], spacing: float,
+                  origin: Sequence[float] = (0.0, 0.0, 0.0),
+                  jitter: float = 0.0, seed: int = 0) -> np.ndarray:
+    """(i, j, k) * spacing + origin in x-major ("ij") order, plus a uniform
+    jitter of +-jitter*spacing per coordinate from ``default_rng(seed)``;
+    float64."""
+    axes = [np.arange(c, dtype=np.float64) * spacing + o for c, o in zip(cells, origin)]
+    xx, yy, zz = np.meshgrid(*axes, indexing="ij")
+    x = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1)
+    if jitter:
+        rng = np.random.default_rng(seed)
+        x = x + rng.uniform(-jitter, jitter, x.shape) * spacing
+    return x
+
+
+def dam_break(cfg: SolverConfig) -> Scene:
+    """Fluid lattice block at ``fluid_origin`` in a box of ``boundary_layers``
+    wall layers."""
+    xb = box_shell(cfg.box, cfg.spacing, cfg.boundary_layers)
+    x = lattice_block(cfg.fluid_cells, cfg.spacing, cfg.fluid_origin, cfg.jitter, cfg.seed)
+    x32 = x.astype(np.float32)
+    xb32 = xb.astype(np.float32)
+    lo, hi = _bounds(x32, xb32)
+    return Scene(x32, xb32, np.zeros_like(xb32), xb32.shape[0], lo, hi)
+
+
+GENERATORS = {
+    "reference_box": reference_box,
+    "dam_break": dam_break,
+}
+
+
+def build(cfg: SolverConfig) -> Scene:
+    if cfg.scene == "torus":
+        from .torus import torus_scene
+        return torus_scene(cfg)
+    try:
+        gen = GENERATORS[cfg.scene]
+    except KeyError:
+        raise ValueError(f"unknown scene {cfg.scene!r}") from None
+    return gen(cfg)
+
+
+def domain_grid_dims(lo: np.ndarray, hi: np.ndarray, h: float, slack_cells: int = 4,
+                     max_dim: int = 1024) -> Tuple[int, int, int]:
+    """Hash dims that give every occupied cell of [lo, hi] its own slot.
+
+    Cell coordinates are trunc(p / h) (float32), so the occupied range on an
+    axis is [trunc(lo/h), trunc(hi/h)]; queries reach one cell further on each
+    side. Points outside the range still work through the modulo hash, they
+    only share slots.
+    """
+    hf = np.float32(h)
+    inv = np.float32(1.0) / hf
+    dims = []
+    for a in range(3):
+        c0 = int(np.trunc(np.float32(lo[a]) * inv))
+        c1 = int(np.trunc(np.float32(hi[a]) * inv))
+        d = c1 - c0 + 1 + 2 + 2 * slack_cells
+        dims.append(max(3, min(d, max_dim)))
+    return tuple(dims)  # type: ignore
+
+
+def grid_dims_for(cfg: SolverConfig, scene: Scene) -> Tuple[Tuple[int, int, int], Tuple[int, int, int]]:
+    """(fluid dims, boundary dims) for the configured grid flavour."""
+    if cfg.grid == "reference":
+        d = cfg.reference_grid_dims()
+        return d, d
+    d = domain_grid_dims(scene.bounds_lo, scene.bounds_hi, cfg.h)
+    return d, d
