@@ -44,11 +44,31 @@ def parse():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--preset", default="dambreak_1m")
-    ap.add_argument("--jacobi", choices=["persistent", "host"], default="persistent")
-    ap.add_argument("--warm-start", choices=["zero", "previous"], default="zero")
+    ap.add_argument("--jacobi", choices=["persistent", "host", "device"], default="device")
+    ap.add_argument("--chunk", type=int, default=0, help="device-loop iterations per chunk")
+    ap.add_argument("--warm-start", choices=["zero", "previous"], default="previous")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--coop-per-cu", type=int, default=0,
+                    help="persistent kernel blocks per CU (0: solver default)")
+    ap.add_argument("--trace-steps", type=int, default=0,
+                    help="print wall time and solver statistics of this many single steps first")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib: int) -> None:
+    """A plain device-to-device copy of known size, to scale the FETCH_SIZE /
+    WRITE_SIZE counters of a profiling run."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": n * 4, "launches": 5}), flush=True)
 
 
 def traffic_from_profile(workload: str):
@@ -94,6 +114,9 @@ def cfg_n(x):
 
 def main():
     args = parse()
+    if args.calibrate_copy_mib:
+        calibrate_copy(args.calibrate_copy_mib)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -103,16 +126,31 @@ def main():
     from warpsph_amd.solver import IISPH
 
     cfg = config.preset(args.preset)
-    cfg = cfg.replace(jacobi_mode=config.JACOBI_PERSISTENT if args.jacobi == "persistent"
-                      else config.JACOBI_HOST,
+    cfg = cfg.replace(jacobi_mode={"persistent": config.JACOBI_PERSISTENT,
+                                   "host": config.JACOBI_HOST,
+                                   "device": config.JACOBI_DEVICE}[args.jacobi],
+                      chunk_iters=args.chunk,
                       warm_start=config.WARM_ZERO if args.warm_start == "zero"
                       else config.WARM_PREVIOUS)
+    if args.coop_per_cu:
+        cfg = cfg.replace(coop_blocks_per_cu=args.coop_per_cu)
     t_setup = time.perf_counter()
     s = IISPH(stage_path=None, config=cfg)
     lib, h = s._lib, s._h
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s n={s.n} nb={s.boundary_n}",
           file=sys.stderr, flush=True)
 
+    for k in range(args.trace_steps):
+        st = N.WsphStepStats()
+        t = time.perf_counter()
+        rc = lib.wsph_step(h, C.byref(st))
+        tm1 = (C.c_double * 8)()
+        print(f"[bench] trace step {k}: rc={rc} {1e3 * (time.perf_counter() - t):.2f} ms "
+              f"iters={st.iterations} residual={st.residual:.3e} vmax={st.v_max:.3f} "
+              f"status={st.status} max_ff={st.max_ff_count} max_fs={st.max_fs_count} "
+              f"counters={s.counters()}",
+              file=sys.stderr, flush=True)
+        N.check("wsph_step", rc)
     for k in range(args.warmup):
         N.check("wsph_step", lib.wsph_step(h, None))
     N.check("wsph_sync", lib.wsph_sync(h))

@@ -45,14 +45,18 @@ enum {
 /* ---- enumerations ------------------------------------------------------ */
 enum { WSPH_LAYOUT_LINEAR = 0, WSPH_LAYOUT_MORTON = 1 };
 /* pressure start value each step: 0 as IISPH.py:166-174, the previous
- * step's pressure, or a caller-supplied field (learned predictor) */
+ * step's pressure, or a caller-supplied field (learned predictor). The
+ * previous step's pressure is carried as zero for particles whose a_ii fell
+ * inside update_p's guard band |a_ii| < INV_SMALL (IISPH.py:535). */
 enum { WSPH_WARM_ZERO = 0, WSPH_WARM_PREVIOUS = 1, WSPH_WARM_EXTERNAL = 2 };
 /* dt: CFL rule of IISPH.py:1368-1370, or fixed */
 enum { WSPH_DT_CFL = 0, WSPH_DT_FIXED = 1 };
 /* Jacobi driver: one launch per kernel with a host convergence test per
- * iteration (the reference's loop, IISPH.py:1200-1296), or one persistent
- * cooperative kernel with a device-side residual */
-enum { WSPH_JACOBI_HOST = 0, WSPH_JACOBI_PERSISTENT = 1 };
+ * iteration (the reference's loop, IISPH.py:1200-1296); one persistent
+ * cooperative kernel with a device-side residual; or launches queued ahead
+ * in chunks whose stop test runs on the device (no host round trip per
+ * iteration) */
+enum { WSPH_JACOBI_HOST = 0, WSPH_JACOBI_PERSISTENT = 1, WSPH_JACOBI_DEVICE = 2 };
 
 /* fields readable with wsph_get_field (original particle order) */
 enum {
@@ -102,7 +106,8 @@ typedef struct wsph_params {
   int max_fs;       /* fluid-boundary neighbour capacity per particle */
   int jacobi_mode;  /* WSPH_JACOBI_* */
   int block;        /* threads per block of the neighbour passes (64..1024) */
-  int coop_blocks_per_cu; /* persistent kernel blocks per CU (0 = occupancy max) */
+  int coop_blocks_per_cu; /* persistent kernel blocks per CU (0 = default 2) */
+  int chunk_iters;  /* WSPH_JACOBI_DEVICE: iterations queued per chunk (0 = 2) */
 } wsph_params;
 
 /* Per-step statistics (what IISPH.step computes and prints, IISPH.py:1200-1371) */
@@ -186,7 +191,8 @@ int wsph_set_timing(wsph_solver *s, int enable);
 /* Cumulative counters since wsph_create, kept on the device so that a run
  * of steps needs no host synchronisation: out[0] steps, out[1] Jacobi
  * iterations, out[2] steps that ended above eta, out[3] A p evaluations,
- * out[4] OR of the device status bits. */
+ * out[4] OR of the device status bits, out[5] most blocks seen at a
+ * persistent-kernel barrier wait that timed out, out[6] persistent grid. */
 int wsph_get_counters(wsph_solver *s, long long *out, int nout);
 
 /* Block until the solver's stream is idle. */

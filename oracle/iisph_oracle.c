@@ -646,6 +646,9 @@ int orc_step(const wsph_params *P, const float *wt, int nw, const float *gt, int
     ORC_COPY(Ap, Ap, 1);
     ORC_COPY(s, s, 3);
 #undef ORC_COPY
+    if (out->p_carry)
+      for (int i = 0; i < n; ++i)
+        out->p_carry[i] = (a_ii[i] > -inv_small && a_ii[i] < inv_small) ? 0.0f : p[i];
     if (out->ff_count) memcpy(out->ff_count, ff.num, sizeof(int) * (size_t)n);
     if (out->fs_count) memcpy(out->fs_count, fs.num, sizeof(int) * (size_t)n);
   }

@@ -14,6 +14,7 @@ extern "C" {
 typedef struct orc_out {
   /* optional per-particle outputs (NULL to skip), original order */
   float *rho, *v_adv, *rho_adv, *d_ii, *a_ii, *p, *Ap, *s;
+  float *p_carry; /* p, zero where |a_ii| < inv_small (start value of the next step) */
   int *ff_count, *fs_count;
   double *res_hist; /* residual after each Jacobi iteration */
   int res_hist_len;

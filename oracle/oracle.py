@@ -41,7 +41,7 @@ class OrcOut(C.Structure):
     _fields_ = [
         ("rho", C.c_void_p), ("v_adv", C.c_void_p), ("rho_adv", C.c_void_p),
         ("d_ii", C.c_void_p), ("a_ii", C.c_void_p), ("p", C.c_void_p),
-        ("Ap", C.c_void_p), ("s", C.c_void_p), ("ff_count", C.c_void_p),
+        ("Ap", C.c_void_p), ("s", C.c_void_p), ("p_carry", C.c_void_p), ("ff_count", C.c_void_p),
         ("fs_count", C.c_void_p), ("res_hist", C.c_void_p), ("res_hist_len", C.c_int),
         ("iterations", C.c_int), ("ap_evals", C.c_int), ("converged", C.c_int),
         ("max_ff", C.c_int), ("max_fs", C.c_int), ("residual", C.c_double),
@@ -127,7 +127,7 @@ def step(params, w_table, g_table, x, v, xb, vb, psi, p0=None, residual_mode: in
     keep = {}
     if want_fields:
         for name, w in (("rho", 1), ("v_adv", 3), ("rho_adv", 1), ("d_ii", 3), ("a_ii", 1),
-                        ("p", 1), ("Ap", 1), ("s", 3)):
+                        ("p", 1), ("Ap", 1), ("s", 3), ("p_carry", 1)):
             a = np.zeros((n, w) if w > 1 else n, dtype=np.float32)
             keep[name] = a
             setattr(out, name, a.ctypes.data)
@@ -167,6 +167,7 @@ class OracleSolver:
         self.xb, self.vb = scene.xb, scene.vb
         self.psi = boundary_psi(self.params, self.wt, self.xb)
         self.p = np.zeros(self.x.shape[0], dtype=np.float32)
+        self.p_carry = self.p.copy()
         self.dt = cfg.dt0
         self.sim_time = 0.0
         self.last = None
@@ -175,10 +176,11 @@ class OracleSolver:
         cfg = self.cfg
         self.params.dt = self.dt
         if p0 is None and cfg.warm_start == 1:
-            p0 = self.p
+            p0 = self.p_carry
         self.x, self.v, info = step(self.params, self.wt, self.gt, self.x, self.v, self.xb,
                                     self.vb, self.psi, p0=p0, want_fields=True)
         self.p = info["p"]
+        self.p_carry = info["p_carry"]
         info["dt"] = self.dt
         self.sim_time += self.dt
         if cfg.dt_mode == 0:

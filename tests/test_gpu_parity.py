@@ -24,7 +24,7 @@ def _announce(msg):
 
 def gpu_solver(cfg, sc=None):
     from warpsph_amd.solver import IISPH
-    return IISPH(stage_path=None, config=cfg, scene=sc)
+    return IISPH(stage_path=None, config=cfg, scene=sc, collect_stats=True)
 
 
 def fields_of(s):
@@ -79,11 +79,13 @@ def test_cell_keys_bitexact(orc):
     assert_same("keys", keys, orc.hash_keys(sc.x, cfg.h, cfg.reference_grid_dims()))
 
 
-@pytest.mark.parametrize("mode", [config.JACOBI_PERSISTENT, config.JACOBI_HOST])
+@pytest.mark.parametrize("mode,chunk", [(config.JACOBI_PERSISTENT, 0), (config.JACOBI_HOST, 0),
+                                        (config.JACOBI_DEVICE, 0), (config.JACOBI_DEVICE, 1),
+                                        (config.JACOBI_DEVICE, 7)])
 @pytest.mark.parametrize("force", [3, 0])
-def test_one_step_bitexact(orc, mode, force):
-    cfg = config.dambreak_8k().replace(jacobi_mode=mode, force_iters=force)
-    _announce(f"one step mode={mode} force={force}")
+def test_one_step_bitexact(orc, mode, chunk, force):
+    cfg = config.dambreak_8k().replace(jacobi_mode=mode, force_iters=force, chunk_iters=chunk)
+    _announce(f"one step mode={mode} chunk={chunk} force={force}")
     s = gpu_solver(cfg)
     s.step()
     sc, P, wt, gt = oracle_setup(cfg)
@@ -92,9 +94,10 @@ def test_one_step_bitexact(orc, mode, force):
     compare_step(s, info, x_o, v_o)
 
 
-@pytest.mark.parametrize("preset,steps", [("dambreak_8k", 25)])
-def test_trajectory_bitexact(orc, preset, steps):
-    cfg = config.preset(preset)
+@pytest.mark.parametrize("preset,steps,mode", [("dambreak_8k", 25, config.JACOBI_DEVICE),
+                                              ("dambreak_8k", 12, config.JACOBI_PERSISTENT)])
+def test_trajectory_bitexact(orc, preset, steps, mode):
+    cfg = config.preset(preset).replace(jacobi_mode=mode, warm_start=config.WARM_PREVIOUS)
     _announce(f"trajectory {preset} {steps} steps")
     s = gpu_solver(cfg)
     sc = scene.build(cfg)
@@ -170,3 +173,26 @@ def test_viscosity_term_bitexact(orc):
     psi = orc.boundary_psi(P, wt, sc.xb)
     x_o, v_o, info = orc.step(P, wt, gt, sc.x, v0, sc.xb, sc.vb, psi)
     compare_step(s, info, x_o, v_o)
+
+
+def test_drivers_agree_at_full_size():
+    """At the 1M-particle size the three Jacobi drivers (host test per
+    iteration, persistent kernel, device-side stop test) give identical
+    states: the arithmetic is the same and the loop stops at the same
+    iteration."""
+    cfg = config.dambreak_1m().replace(warm_start=config.WARM_PREVIOUS)
+    out = {}
+    for mode in (config.JACOBI_DEVICE, config.JACOBI_PERSISTENT, config.JACOBI_HOST):
+        _announce(f"1M drivers mode={mode}")
+        s = gpu_solver(cfg.replace(jacobi_mode=mode))
+        its = []
+        for _ in range(3):
+            s.step()
+            its.append(s.step_stats()["iterations"])
+        out[mode] = (its, s.x.cpu().numpy(), s.v.cpu().numpy(), s.p.cpu().numpy())
+        s.close()
+    ref = out[config.JACOBI_DEVICE]
+    for mode, got in out.items():
+        assert got[0] == ref[0], (mode, got[0], ref[0])
+        for a, b in zip(got[1:], ref[1:]):
+            assert_same(f"mode {mode}", a, b)

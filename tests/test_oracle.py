@@ -276,3 +276,18 @@ def test_fixed_point_residual_tracks_float_sum(orc):
     _, _, a = orc.step(P, wt, gt, sc.x, v, sc.xb, sc.vb, psi, residual_mode=0)
     _, _, b = orc.step(P, wt, gt, sc.x, v, sc.xb, sc.vb, psi, residual_mode=1)
     assert np.allclose(a["res_hist"], b["res_hist"], rtol=1e-4)
+
+
+@pytest.mark.parametrize("name", ["reference", "dambreak_8k"])
+def test_scene_arrays_are_row_major(name):
+    """The device handoff reads raw row-major float32 memory."""
+    sc = scene.build(config.preset(name))
+    for a in (sc.x, sc.xb, sc.vb):
+        assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+
+
+def test_reference_scene_counts():
+    """IISPH.init_particles for the reference's EMPTY scene: 77x31x43 fluid and
+    a 3-layer shell of (80+6)^3 - 80^3 wall particles."""
+    sc = scene.build(config.reference())
+    assert (sc.n, sc.nb) == (102641, 124056)

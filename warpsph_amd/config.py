@@ -21,6 +21,7 @@ WARM_PREVIOUS = 1
 WARM_EXTERNAL = 2
 JACOBI_HOST = 0
 JACOBI_PERSISTENT = 1
+JACOBI_DEVICE = 2
 LAYOUT_LINEAR = 0
 LAYOUT_MORTON = 1
 
@@ -63,11 +64,12 @@ class SolverConfig:
     # device layout
     grid: str = "domain"           # "domain" (covering) or "reference" (int(BOX/h/4))
     cell_layout: int = LAYOUT_MORTON
-    max_ff: int = 64
-    max_fs: int = 64
-    jacobi_mode: int = JACOBI_PERSISTENT
+    max_ff: int = 128
+    max_fs: int = 96
+    jacobi_mode: int = JACOBI_DEVICE
     block: int = 256
     coop_blocks_per_cu: int = 0
+    chunk_iters: int = 0
     extra: dict = field(default_factory=dict)
 
     # ---- derived quantities (wp_sph/main.py:57-60) ---------------------

@@ -103,12 +103,14 @@ __global__ void k_set_dt(DevState *st, double dt, double t) {
   st->dt = dt;
   st->sim_time = t;
 }
-__global__ void k_accumulate_totals(DevState *st, long long *tot) {
+__global__ void k_accumulate_totals(DevState *st, long long *tot, int grid) {
   tot[0] += 1;                  // steps
   tot[1] += st->iterations;     // Jacobi iterations
   tot[2] += st->converged ? 0 : 1;
   tot[3] += st->ap_evals;
   tot[4] |= st->status;
+  tot[5] = tot[5] > (long long)st->bar_seen ? tot[5] : (long long)st->bar_seen;
+  tot[6] = grid;
 }
 
 static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -137,10 +139,12 @@ struct wsph_solver {
   int *orig[2] = {nullptr, nullptr};
   int cur = 0;
   float4 *vadv = nullptr, *dp[2] = {nullptr, nullptr}, *sap = nullptr, *aux = nullptr;
-  float *ap = nullptr, *pstart = nullptr, *p_ext = nullptr;
+  float *ap = nullptr, *pstart = nullptr, *p_ext = nullptr, *pfinal = nullptr;
+  unsigned *vmax_part = nullptr;
   bool have_p_ext = false;
   int *slot = nullptr, *rank = nullptr, *tmp = nullptr, *perm = nullptr;
   int *nbr_ff = nullptr, *cnt_ff = nullptr, *nbr_fs = nullptr, *cnt_fs = nullptr;
+  float *sg_ff = nullptr, *sg_fs = nullptr;
   GridBufs fg, bg;
   // boundary
   float4 *bpos = nullptr, *bvel = nullptr;
@@ -155,6 +159,10 @@ struct wsph_solver {
   long t_recorded = 0, t_processed = 0;
   double t_sum[kEv] = {0};
   int persistent_grid = 0;
+  // device-driven Jacobi loop: pinned copies of DevState::done_tag per chunk
+  int *done_flags = nullptr;
+  hipEvent_t chunk_ev[2] = {};
+  int step_count = 0;
 };
 
 static long grid_cells(const int *dims, int layout) {
@@ -199,12 +207,16 @@ static Bufs make_bufs(wsph_solver *s) {
   b.ap = s->ap;
   b.pprev = s->pprev[s->cur];
   b.pstart = s->pstart;
+  b.pfinal = s->pfinal;
+  b.vmax_part = s->vmax_part;
   b.fstart = s->fg.start;
   b.fcount = s->fg.count;
   b.nbr_ff = s->nbr_ff;
   b.cnt_ff = s->cnt_ff;
   b.nbr_fs = s->nbr_fs;
   b.cnt_fs = s->cnt_fs;
+  b.sg_ff = s->sg_ff;
+  b.sg_fs = s->sg_fs;
   b.cap_ff = s->P.max_ff;
   b.cap_fs = s->P.max_fs;
   b.bpos = s->bpos;
@@ -280,6 +292,9 @@ int wsph_create(const wsph_params *params, wsph_solver **out) {
   CREATE_TRY(hipMemsetAsync(s->totals, 0, sizeof(long long) * 8, s->stream));
   CREATE_TRY(hipMemsetAsync(s->st, 0, sizeof(DevState), s->stream));
   for (auto &e : s->ev) CREATE_TRY(hipEventCreate(&e));
+  for (auto &e : s->chunk_ev) CREATE_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  CREATE_TRY(hipHostMalloc((void **)&s->done_flags, sizeof(int) * (kMaxIters + 8),
+                           hipHostMallocDefault));
   k_set_dt<<<1, 1, 0, s->stream>>>(s->st, P.dt, 0.0);
   CREATE_TRY(hipGetLastError());
   int rcg = alloc_grid(s->fg, P.grid_dims, P.cell_layout);
@@ -308,6 +323,8 @@ static void free_fluid(wsph_solver *s) {
   dfree(s->ap);
   dfree(s->pstart);
   dfree(s->p_ext);
+  dfree(s->pfinal);
+  dfree(s->vmax_part);
   dfree(s->slot);
   dfree(s->rank);
   dfree(s->tmp);
@@ -316,6 +333,8 @@ static void free_fluid(wsph_solver *s) {
   dfree(s->cnt_ff);
   dfree(s->nbr_fs);
   dfree(s->cnt_fs);
+  dfree(s->sg_ff);
+  dfree(s->sg_fs);
   s->n = 0;
 }
 
@@ -338,6 +357,9 @@ int wsph_destroy(wsph_solver *s) {
   }
   for (auto &e : s->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : s->chunk_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->done_flags) (void)hipHostFree(s->done_flags);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return WSPH_OK;
@@ -450,6 +472,8 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
     HIP_TRY(dalloc(&s->aux, n));
     HIP_TRY(dalloc(&s->ap, n));
     HIP_TRY(dalloc(&s->pstart, n));
+    HIP_TRY(dalloc(&s->pfinal, n));
+    HIP_TRY(dalloc(&s->vmax_part, (size_t)(n + 63) / 64 + 1));
     HIP_TRY(dalloc(&s->p_ext, n));
     HIP_TRY(dalloc(&s->slot, n));
     HIP_TRY(dalloc(&s->rank, n));
@@ -459,6 +483,8 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
     HIP_TRY(dalloc(&s->cnt_fs, n));
     HIP_TRY(dalloc(&s->nbr_ff, (size_t)n * s->P.max_ff));
     HIP_TRY(dalloc(&s->nbr_fs, (size_t)n * s->P.max_fs));
+    HIP_TRY(dalloc(&s->sg_ff, (size_t)n * s->P.max_ff));
+    HIP_TRY(dalloc(&s->sg_fs, (size_t)n * s->P.max_fs));
     s->cap_n = n;
   }
   s->n = n;
@@ -470,6 +496,7 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
     k_iota<<<nblk(n, 256), 256, 0, s->stream>>>(s->orig[0], n);
     k_fill_f32<<<nblk(n, 256), 256, 0, s->stream>>>(s->pprev[0], n, 0.0f);
     k_fill_f32<<<nblk(n, 256), 256, 0, s->stream>>>(s->p_ext, n, 0.0f);
+    k_fill_f32<<<nblk(n, 256), 256, 0, s->stream>>>(s->pfinal, n, 0.0f);
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -508,6 +535,7 @@ int wsph_neighbor_search(wsph_solver *s) {
   if (!s || !s->pos[0]) return fail(WSPH_E_ARG, "no fluid set");
   if (s->timing && s->t_recorded - s->t_processed >= wsph_solver::kRing) timing_fold_one(s);
   launch_reset_state(make_bufs(s), s->stream);
+  s->step_count++;
   rec(s, 0);
   const int c = s->cur, o = c ^ 1;
   int rc = sort_points(s, s->fg, s->pos[c], s->orig[c], s->n, s->slot, s->rank, s->tmp, s->perm);
@@ -552,11 +580,17 @@ int wsph_pressure_solve(wsph_solver *s, wsph_step_stats *stats) {
   }
   if (P.jacobi_mode == WSPH_JACOBI_PERSISTENT) {
     int per = 0;
-    int maxb = persistent_max_blocks(b, s->C, P.block, &per);
-    if (maxb <= 0) return fail(WSPH_E_UNSUPPORTED, "persistent kernel does not fit on the device");
-    if (P.coop_blocks_per_cu > 0 && P.coop_blocks_per_cu < per) maxb = maxb / per * P.coop_blocks_per_cu;
+    const int maxb = persistent_max_blocks(b, s->C, P.block, &per);
+    if (maxb <= 0 || per <= 0)
+      return fail(WSPH_E_UNSUPPORTED, "persistent kernel does not fit on the device");
+    // The occupancy figure is not a co-residency guarantee (a launch at the
+    // full figure has left blocks unscheduled), so stay at half of it.
+    int eff = P.coop_blocks_per_cu > 0 ? P.coop_blocks_per_cu : 2;
+    const int cap = per / 2 > 0 ? per / 2 : 1;
+    eff = eff < cap ? eff : cap;
+    const int ncu = maxb / per;
     const int need = (int)nblk(s->n, P.block);
-    const int grid = need < maxb ? need : maxb;
+    const int grid = need < eff * ncu ? need : eff * ncu;
     s->persistent_grid = grid;
     if (s->n > 0) {
       hipError_t e = launch_jacobi_persistent(b, s->C, P, src, grid > 0 ? grid : 1, P.block, s->stream);
@@ -564,6 +598,27 @@ int wsph_pressure_solve(wsph_solver *s, wsph_step_stats *stats) {
         return fail(WSPH_E_UNSUPPORTED, "cooperative launch failed: %s", hipGetErrorString(e));
     }
     s->final_dp = 0;
+  } else if (P.jacobi_mode == WSPH_JACOBI_DEVICE) {
+    // iterations queued in chunks, one chunk ahead of the host's look at the
+    // device's stop flag
+    const int C = P.chunk_iters > 0 ? P.chunk_iters : 2;
+    const int max_it = P.force_iters > 0 ? P.force_iters : P.max_iters;
+    const int tag = s->step_count;
+    int queued = 0;
+    for (int m = 0;; ++m) {
+      const int cnt = C < max_it - queued ? C : max_it - queued;
+      launch_jacobi_chunk(b, s->C, P, src, cnt, P.block, s->stream);
+      queued += cnt;
+      HIP_TRY(hipMemcpyAsync(&s->done_flags[m], &s->st->done_tag, sizeof(int),
+                             hipMemcpyDeviceToHost, s->stream));
+      HIP_TRY(hipEventRecord(s->chunk_ev[m & 1], s->stream));
+      if (queued >= max_it) break;
+      if (m >= 1) {
+        HIP_TRY(hipEventSynchronize(s->chunk_ev[(m - 1) & 1]));
+        if (__atomic_load_n(&s->done_flags[m - 1], __ATOMIC_ACQUIRE) == tag) break;
+      }
+    }
+    s->final_dp = -1;
   } else {
     // one launch per kernel and a host convergence test each iteration,
     // as the reference's while loop (IISPH.py:1200-1296)
@@ -621,9 +676,9 @@ int wsph_integrate(wsph_solver *s, wsph_step_stats *stats) {
     HIP_TRY(hipMemcpyAsync(&s->st_host->dt, &s->st->dt, sizeof(double), hipMemcpyDeviceToHost,
                            s->stream));
   }
-  launch_integrate(b, s->C, s->final_dp, s->P.block, s->stream);
+  launch_integrate(b, s->C, s->final_dp, s->P.block, s->stream);  // -1: index kept on device
   launch_clock(b, s->P, s->stream);
-  k_accumulate_totals<<<1, 1, 0, s->stream>>>(s->st, s->totals);
+  k_accumulate_totals<<<1, 1, 0, s->stream>>>(s->st, s->totals, s->persistent_grid);
   rec(s, 4);
   HIP_TRY(hipGetLastError());
   s->searched = s->advected = s->solved = false;
@@ -652,10 +707,15 @@ int wsph_get_field(wsph_solver *s, int field, void *dst) {
   const int n = s->n;
   const int *orig = s->orig[s->cur];
   const int c = s->cur;
+  if (s->final_dp < 0) {
+    HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->final_dp = s->st_host->final_src;
+  }
   switch (field) {
     case WSPH_F_X: launch_scatter(s->pos[c], 4, 3, 0, orig, n, dst, s->stream); break;
     case WSPH_F_V: launch_scatter(s->vel[c], 4, 3, 0, orig, n, dst, s->stream); break;
-    case WSPH_F_P: launch_scatter(s->pprev[c], 1, 1, 0, orig, n, dst, s->stream); break;
+    case WSPH_F_P: launch_scatter(s->pfinal, 1, 1, 0, orig, n, dst, s->stream); break;
     case WSPH_F_RHO: launch_scatter(s->pos[c], 4, 1, 3, orig, n, dst, s->stream); break;
     case WSPH_F_RHO_ADV: launch_scatter(s->aux, 4, 1, 0, orig, n, dst, s->stream); break;
     case WSPH_F_V_ADV: launch_scatter(s->vadv, 4, 3, 0, orig, n, dst, s->stream); break;

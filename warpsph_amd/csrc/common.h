@@ -49,7 +49,13 @@ struct DevState {
   int max_ff, max_fs;         // largest neighbour counts seen
   int status;                 // ST_* bits
   int done;                   // Jacobi loop finished flag
+  int iter;                   // current Jacobi iteration (device-driven loop)
+  int final_src;              // dp buffer holding the final iterate
+  int step_tag;               // incremented by every step's state reset
+  int done_tag;               // step_tag of the step whose Jacobi loop finished last
+  unsigned blocks_done;       // blocks of the running J2 that have finished
   unsigned bar_count, bar_gen;  // grid barrier
+  unsigned bar_seen;            // arrivals seen by a barrier wait that timed out
   unsigned long long res_sum[kMaxIters];
   unsigned res_cnt[kMaxIters];
 };
@@ -139,6 +145,39 @@ __device__ __forceinline__ const float *stage_table(float *lds, const float *g, 
   for (int k = threadIdx.x; k < n; k += blockDim.x) lds[k] = g[k];
   __syncthreads();
   return lds;
+}
+
+// a / b rounded to nearest, given y = 1.0f / b rounded to nearest: one
+// Markstein correction step, q0 = a*y, r = a - q0*b (exact with fma),
+// q = q0 + r*y. Equals IEEE a / b for normal operands (checked against the
+// hardware division for every float32 significand of b); used where one
+// divisor serves many quotients.
+__device__ __forceinline__ float div_by(float a, float b, float y) {
+  const float q0 = a * y;
+  const float r = __builtin_fmaf(-q0, b, a);
+  return __builtin_fmaf(r, y, q0);
+}
+
+// Visit the neighbours of particle i from a column-major list in order,
+// loading kBatch indices and their gathers before evaluating them, so that
+// several memory round trips are in flight per thread. `load(j, slot)`
+// returns the data of neighbour j (slot = its list position k * n + i, for
+// per-pair arrays), `body(data)` accumulates in list order.
+template <int kBatch, typename LoadF, typename BodyF>
+__device__ __forceinline__ void for_neighbors(const int *__restrict__ list, int n, int i, int cnt,
+                                              LoadF load, BodyF body) {
+  int k = 0;
+  for (; k + kBatch <= cnt; k += kBatch) {
+    int j[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) j[u] = list[(long)(k + u) * n + i];
+    decltype(load(0, 0L)) d[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) d[u] = load(j[u], (long)(k + u) * n + i);
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) body(d[u]);
+  }
+  for (; k < cnt; ++k) body(load(list[(long)k * n + i], (long)k * n + i));
 }
 
 // Fixed-point residual term of one particle (WSPH_RESIDUAL_* in wsph.h).

@@ -41,12 +41,15 @@ struct Bufs {
   float4 *sap;      // s = sum_j d_ij p_j (xyz), first part of A p (w)
   float4 *aux;      // rho_adv, a_ii
   float *ap;        // A p
-  float *pprev;     // pressure carried to the next step
+  float *pprev;     // pressure carried to the next step's start value
+  float *pfinal;    // final pressure of the last step
+  unsigned *vmax_part;  // per-block max |v| of k_integrate
   const float *pstart;  // caller-supplied start pressure, sorted order (or null)
   // fluid grid
   const unsigned *fstart, *fcount;
   // neighbour lists, column-major: list[k * n + i]
   int *nbr_ff, *cnt_ff, *nbr_fs, *cnt_fs;
+  float *sg_ff, *sg_fs;  // per-pair grad-W table value, same layout as the lists
   int cap_ff, cap_fs;
   // boundary (sorted by its own grid)
   float4 *bpos;     // x, y, z, psi
@@ -68,6 +71,10 @@ void launch_assemble(const Bufs &b, const SphConst &c, int start_mode, int block
 // accumulates the residual of iteration `slot`.
 void launch_j1(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s);
 void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, hipStream_t s);
+// `iters` Jacobi iterations with device-side stop test (no-ops once done)
+void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P, int src0,
+                         int iters, int block, hipStream_t s);
+// src < 0: read the final buffer index from DevState
 void launch_integrate(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s);
 void launch_clock(const Bufs &b, const wsph_params &P, hipStream_t s);
 void launch_reset_state(const Bufs &b, hipStream_t s);

@@ -29,6 +29,36 @@ __device__ __forceinline__ Tabs stage_tabs(const Bufs &b, const SphConst &c) {
   }
 }
 
+constexpr int kBatch = 4;  // neighbours loaded ahead per thread
+
+struct F4x2 {
+  float4 a, b;
+};
+struct G1 {  // neighbour data + the pair's grad-W table value
+  float4 a;
+  float s;
+};
+struct G2 {
+  float4 a, b;
+  float s;
+};
+
+// grad W_ij from the pair's stored table value s = G(|r|) (or 0), r = x_j - x_i:
+// the last step of grad_spline_W (sph_funcs.py:118)
+__device__ __forceinline__ float3 grad_from(float s, float3 r) {
+  return make_float3(s * r.x, s * r.y, s * r.z);
+}
+
+// table value of grad_spline_W at |r| = len (sph_funcs.py:105-117)
+__device__ __forceinline__ float grad_scalar(const float *G, int n, float scale, float h, float len) {
+  if (len >= h || len < 1e-10f) return 0.0f;
+  const float idx = len * scale;
+  int lo = (int)idx;
+  lo = lo < n - 2 ? lo : n - 2;
+  const float a = idx - (float)lo;
+  return (1.0f - a) * G[lo] + a * G[lo + 1];
+}
+
 #define W_OF(T, r) spline_w((T).w, c.nw, c.w_scale, c.h, (r))
 #define GW_OF(T, r) grad_w((T).g, c.ng, c.g_scale, c.h, (r))
 
@@ -45,70 +75,88 @@ __device__ __forceinline__ float update_p(float p, float rho_adv, float ap, floa
   return pi > 0.0f ? pi : 0.0f;
 }
 
-// ---- compute_boundary_density, IISPH.py:23-47 --------------------------
-template <bool kLds>
-__global__ void k_boundary_psi(Bufs b, SphConst c) {
-  const Tabs T = stage_tabs<kLds>(b, c);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= b.nb) return;
-  const float4 p = b.bpos[i];
-  float rho = W_OF(T, 0.0f);
-  const Range3 q = query_range(p.x, p.y, p.z, c.h, c.cell_inv, c.bdims);
-  for (int z = q.z0; z <= q.z1; ++z)
-    for (int y = q.y0; y <= q.y1; ++y)
-      for (int x = q.x0; x <= q.x1; ++x) {
-        const int sl = cell_slot(hash_axis(x, c.bdims[0]), hash_axis(y, c.bdims[1]),
-                                 hash_axis(z, c.bdims[2]), c.bdims, c.layout);
-        const int s = (int)b.bstart[sl], e = s + (int)b.bcount[sl];
-        for (int u = s; u < e; ++u) {
-          const float4 o = b.bpos[u];
-          const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
-          if (d < c.h && u != i) rho += W_OF(T, d);
-        }
-      }
-  reinterpret_cast<float *>(&b.bpos[i])[3] = c.rho0 / rho;
-}
-
 // ---- count_/store_same_neighbor (IISPH.py:52-104) and
 //      count_/store_diff_neighbor (IISPH.py:107-163) --------------------------
+// Candidates of one query (hash_grid_query, cells x innermost, then y, then
+// z; ascending original index inside a cell) are visited through `visit`.
+// With the linear cell layout the x-cells of one (y, z) row occupy
+// consecutive slots, so their particles form one contiguous range and a row
+// is a single span; otherwise each cell is its own span.
+template <typename VisitF>
+__device__ __forceinline__ void for_query_spans(float px, float py, float pz, const SphConst &c,
+                                                const int *dims, const unsigned *__restrict__ start,
+                                                const unsigned *__restrict__ count, VisitF visit) {
+  const Range3 q = query_range(px, py, pz, c.h, c.cell_inv, dims);
+  for (int z = q.z0; z <= q.z1; ++z) {
+    const int hz = hash_axis(z, dims[2]);
+    for (int y = q.y0; y <= q.y1; ++y) {
+      const int hy = hash_axis(y, dims[1]);
+      const int hx0 = hash_axis(q.x0, dims[0]);
+      const int hx1 = hash_axis(q.x1, dims[0]);
+      if (c.layout == WSPH_LAYOUT_LINEAR && hx1 - hx0 == q.x1 - q.x0) {
+        const int s0 = cell_slot(hx0, hy, hz, dims, c.layout);
+        const int s1 = s0 + (hx1 - hx0);
+        visit((int)start[s0], (int)(start[s1] + count[s1]));
+      } else {
+        for (int x = q.x0; x <= q.x1; ++x) {
+          const int sl = cell_slot(hash_axis(x, dims[0]), hy, hz, dims, c.layout);
+          visit((int)start[sl], (int)(start[sl] + count[sl]));
+        }
+      }
+    }
+  }
+}
+
+// Candidates [s, e) tested kBatch at a time (positions loaded ahead).
+template <typename TestF>
+__device__ __forceinline__ void for_span(const float4 *__restrict__ pts, int s, int e, TestF test) {
+  int j = s;
+  for (; j + kBatch <= e; j += kBatch) {
+    float4 o[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) o[u] = pts[j + u];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) test(j + u, o[u]);
+  }
+  for (; j < e; ++j) test(j, pts[j]);
+}
+
+// Also stores, per pair, the grad-W table value at the pair distance: the
+// distance |x_i - x_j| of the neighbour test equals |x_j - x_i| of
+// grad_spline_W bit for bit, so later passes rebuild grad W_ij as s * r.
+template <bool kLds>
 __global__ void k_lists(Bufs b, SphConst c) {
+  const Tabs T = stage_tabs<kLds>(b, c);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int nf = 0, ns = 0;
   if (i < b.n) {
     const float4 p = b.pos[i];
     const int n = b.n;
-    Range3 q = query_range(p.x, p.y, p.z, c.h, c.cell_inv, c.fdims);
-    for (int z = q.z0; z <= q.z1; ++z)
-      for (int y = q.y0; y <= q.y1; ++y)
-        for (int x = q.x0; x <= q.x1; ++x) {
-          const int sl = cell_slot(hash_axis(x, c.fdims[0]), hash_axis(y, c.fdims[1]),
-                                   hash_axis(z, c.fdims[2]), c.fdims, c.layout);
-          const int s = (int)b.fstart[sl], e = s + (int)b.fcount[sl];
-          for (int j = s; j < e; ++j) {
-            if (j == i) continue;
-            const float4 o = b.pos[j];
-            if (dist3(p.x, p.y, p.z, o.x, o.y, o.z) < c.h) {
-              if (nf < b.cap_ff) b.nbr_ff[(long)nf * n + i] = j;
-              ++nf;
-            }
+    for_query_spans(p.x, p.y, p.z, c, c.fdims, b.fstart, b.fcount, [&](int s0, int e0) {
+      for_span(b.pos, s0, e0, [&](int j, const float4 &o) {
+        const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
+        if (j != i && d < c.h) {
+          if (nf < b.cap_ff) {
+            b.nbr_ff[(long)nf * n + i] = j;
+            b.sg_ff[(long)nf * n + i] = grad_scalar(T.g, c.ng, c.g_scale, c.h, d);
           }
+          ++nf;
         }
-    q = query_range(p.x, p.y, p.z, c.h, c.cell_inv, c.bdims);
+      });
+    });
     if (b.nb > 0)
-      for (int z = q.z0; z <= q.z1; ++z)
-        for (int y = q.y0; y <= q.y1; ++y)
-          for (int x = q.x0; x <= q.x1; ++x) {
-            const int sl = cell_slot(hash_axis(x, c.bdims[0]), hash_axis(y, c.bdims[1]),
-                                     hash_axis(z, c.bdims[2]), c.bdims, c.layout);
-            const int s = (int)b.bstart[sl], e = s + (int)b.bcount[sl];
-            for (int j = s; j < e; ++j) {
-              const float4 o = b.bpos[j];
-              if (dist3(p.x, p.y, p.z, o.x, o.y, o.z) < c.h) {
-                if (ns < b.cap_fs) b.nbr_fs[(long)ns * n + i] = j;
-                ++ns;
-              }
+      for_query_spans(p.x, p.y, p.z, c, c.bdims, b.bstart, b.bcount, [&](int s0, int e0) {
+        for_span(b.bpos, s0, e0, [&](int j, const float4 &o) {
+          const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
+          if (d < c.h) {
+            if (ns < b.cap_fs) {
+              b.nbr_fs[(long)ns * n + i] = j;
+              b.sg_fs[(long)ns * n + i] = grad_scalar(T.g, c.ng, c.g_scale, c.h, d);
             }
+            ++ns;
           }
+        });
+      });
     b.cnt_ff[i] = nf < b.cap_ff ? nf : b.cap_ff;
     b.cnt_fs[i] = ns < b.cap_fs ? ns : b.cap_fs;
   }
@@ -119,6 +167,23 @@ __global__ void k_lists(Bufs b, SphConst c) {
     if (mf > b.cap_ff) atomicOr(&b.st->status, (int)ST_FF_OVERFLOW);
     if (ms > b.cap_fs) atomicOr(&b.st->status, (int)ST_FS_OVERFLOW);
   }
+}
+
+// ---- compute_boundary_density, IISPH.py:23-47 --------------------------
+template <bool kLds>
+__global__ void k_boundary_psi(Bufs b, SphConst c) {
+  const Tabs T = stage_tabs<kLds>(b, c);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.nb) return;
+  const float4 p = b.bpos[i];
+  float rho = W_OF(T, 0.0f);
+  for_query_spans(p.x, p.y, p.z, c, c.bdims, b.bstart, b.bcount, [&](int s0, int e0) {
+    for_span(b.bpos, s0, e0, [&](int u, const float4 &o) {
+      const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
+      if (d < c.h && u != i) rho += W_OF(T, d);
+    });
+  });
+  reinterpret_cast<float *>(&b.bpos[i])[3] = c.rho0 / rho;
 }
 
 // v_adv of predict_v_adv (IISPH.py:217-278) with the Laplacian term switched
@@ -138,16 +203,15 @@ __global__ void k_density(Bufs b, SphConst c, int fuse_vadv) {
   if (i >= b.n) return;
   const int n = b.n;
   const float4 p = b.pos[i];
+  const float4 *__restrict__ pos = b.pos;
+  const float4 *__restrict__ bpos = b.bpos;
   float t1 = 0.0f, t2 = 0.0f;
-  const int nf = b.cnt_ff[i], ns = b.cnt_fs[i];
-  for (int k = 0; k < nf; ++k) {
-    const float4 o = b.pos[b.nbr_ff[(long)k * n + i]];
-    t1 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z));
-  }
-  for (int k = 0; k < ns; ++k) {
-    const float4 o = b.bpos[b.nbr_fs[(long)k * n + i]];
-    t2 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z)) * o.w;
-  }
+  for_neighbors<kBatch>(
+      b.nbr_ff, n, i, b.cnt_ff[i], [&](int j, long) { return pos[j]; },
+      [&](const float4 &o) { t1 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z)); });
+  for_neighbors<kBatch>(
+      b.nbr_fs, n, i, b.cnt_fs[i], [&](int j, long) { return bpos[j]; },
+      [&](const float4 &o) { t2 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z)) * o.w; });
   const float rho = t1 * c.mass + t2 + W_OF(T, 0.0f) * c.mass;
   reinterpret_cast<float *>(&b.pos[i])[3] = rho;
   if (fuse_vadv) b.vadv[i] = vadv_plain(b.vel[i], rho, (float)b.st->dt, c);
@@ -164,34 +228,41 @@ __global__ void k_vadv(Bufs b, SphConst c) {
   const float4 p = b.pos[i];
   const float4 vi = b.vel[i];
   const int nf = b.cnt_ff[i], ns = b.cnt_fs[i];
+  const float4 *__restrict__ pos = b.pos;
+  const float4 *__restrict__ vel = b.vel;
+  const float4 *__restrict__ bpos = b.bpos;
+  const float *__restrict__ sgf = b.sg_ff;
+  const float *__restrict__ sgs = b.sg_fs;
   float lx = 0.0f, ly = 0.0f, lz = 0.0f;
   if (c.vis_mu != 0.0f) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
-    for (int k = 0; k < nf; ++k) {
-      const int j = b.nbr_ff[(long)k * n + i];
-      const float4 o = b.pos[j];
-      const float4 vj = b.vel[j];
-      const float3 xij = sub3(o, p);
-      const float3 gw = GW_OF(T, xij);
-      const float3 vd = make_float3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z);
-      const float dd = dot3(vd, xij);
-      const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
-      const float r2 = d * d;
-      ax = ax - gw.x * dd / o.w / r2;
-      ay = ay - gw.y * dd / o.w / r2;
-      az = az - gw.z * dd / o.w / r2;
-    }
-    for (int k = 0; k < ns; ++k) {
-      const float4 o = b.bpos[b.nbr_fs[(long)k * n + i]];
-      const float3 xib = sub3(o, p);
-      const float3 gw = GW_OF(T, xib);
-      const float dd = dot3(make_float3(vi.x, vi.y, vi.z), xib);
-      const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
-      const float r2 = d * d;
-      bx = bx - gw.x * o.w * dd / r2;
-      by = by - gw.y * o.w * dd / r2;
-      bz = bz - gw.z * o.w * dd / r2;
-    }
+    for_neighbors<kBatch>(
+        b.nbr_ff, n, i, nf, [&](int j, long q) { return G2{pos[j], vel[j], sgf[q]}; },
+        [&](const G2 &v) {
+          const float4 o = v.a, vj = v.b;
+          const float3 xij = sub3(o, p);
+          const float3 gw = grad_from(v.s, xij);
+          const float3 vd = make_float3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z);
+          const float dd = dot3(vd, xij);
+          const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
+          const float r2 = d * d;
+          ax = ax - gw.x * dd / o.w / r2;
+          ay = ay - gw.y * dd / o.w / r2;
+          az = az - gw.z * dd / o.w / r2;
+        });
+    for_neighbors<kBatch>(
+        b.nbr_fs, n, i, ns, [&](int j, long q) { return G1{bpos[j], sgs[q]}; },
+        [&](const G1 &v) {
+          const float4 o = v.a;
+          const float3 xib = sub3(o, p);
+          const float3 gw = grad_from(v.s, xib);
+          const float dd = dot3(make_float3(vi.x, vi.y, vi.z), xib);
+          const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
+          const float r2 = d * d;
+          bx = bx - gw.x * o.w * dd / r2;
+          by = by - gw.y * o.w * dd / r2;
+          bz = bz - gw.z * o.w * dd / r2;
+        });
     lx = 10.0f * (ax * c.mass + bx / c.rho0);
     ly = 10.0f * (ay * c.mass + by / c.rho0);
     lz = 10.0f * (az * c.mass + bz / c.rho0);
@@ -200,16 +271,17 @@ __global__ void k_vadv(Bufs b, SphConst c) {
   float vy = vi.y + (ly * c.vis_mu / p.w + c.g_y) * dt;
   float vz = vi.z + (lz * c.vis_mu / p.w + 0.0f) * dt;
   if (c.xsph != 0.0f) {
+    // XSPH: v_adv += xsph * sum_j (v_j - v_i) * ((m / rho_j) * W_ij)
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for (int k = 0; k < nf; ++k) {
-      const int j = b.nbr_ff[(long)k * n + i];
-      const float4 o = b.pos[j];
-      const float4 vj = b.vel[j];
-      const float coef = c.mass / o.w * W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z));
-      ax = ax + (vj.x - vi.x) * coef;
-      ay = ay + (vj.y - vi.y) * coef;
-      az = az + (vj.z - vi.z) * coef;
-    }
+    for_neighbors<kBatch>(
+        b.nbr_ff, n, i, nf, [&](int j, long) { return F4x2{pos[j], vel[j]}; },
+        [&](const F4x2 &v) {
+          const float4 o = v.a, vj = v.b;
+          const float coef = c.mass / o.w * W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z));
+          ax = ax + (vj.x - vi.x) * coef;
+          ay = ay + (vj.y - vi.y) * coef;
+          az = az + (vj.z - vi.z) * coef;
+        });
     vx = vx + ax * c.xsph;
     vy = vy + ay * c.xsph;
     vz = vz + az * c.xsph;
@@ -231,46 +303,55 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
   const float4 p = b.pos[i];
   const float4 va = b.vadv[i];
   const float ri2 = p.w * p.w;
+  const float yi = 1.0f / ri2;
   const int nf = b.cnt_ff[i], ns = b.cnt_fs[i];
+  const float4 *__restrict__ pos = b.pos;
+  const float4 *__restrict__ vadv = b.vadv;
+  const float4 *__restrict__ bpos = b.bpos;
+  const float4 *__restrict__ bvel = b.bvel;
   float tr1 = 0.0f, tr2 = 0.0f;
   float dx1 = 0.0f, dy1 = 0.0f, dz1 = 0.0f, dx2 = 0.0f, dy2 = 0.0f, dz2 = 0.0f;
-  for (int k = 0; k < nf; ++k) {
-    const int j = b.nbr_ff[(long)k * n + i];
-    const float4 o = b.pos[j];
-    const float4 vj = b.vadv[j];
-    const float3 gw = GW_OF(T, sub3(o, p));
-    tr1 += dot3(make_float3(va.x - vj.x, va.y - vj.y, va.z - vj.z), gw);
-    dx1 = dx1 - gw.x;
-    dy1 = dy1 - gw.y;
-    dz1 = dz1 - gw.z;
-  }
-  for (int k = 0; k < ns; ++k) {
-    const int j = b.nbr_fs[(long)k * n + i];
-    const float4 o = b.bpos[j];
-    const float4 vb = b.bvel[j];
-    const float3 gw = GW_OF(T, sub3(o, p));
-    tr2 += dot3(make_float3(va.x - vb.x, va.y - vb.y, va.z - vb.z), gw) * o.w;
-    dx2 = dx2 - gw.x * o.w;
-    dy2 = dy2 - gw.y * o.w;
-    dz2 = dz2 - gw.z * o.w;
-  }
+  const float *__restrict__ sgf = b.sg_ff;
+  const float *__restrict__ sgs = b.sg_fs;
+  for_neighbors<kBatch>(
+      b.nbr_ff, n, i, nf, [&](int j, long q) { return G2{pos[j], vadv[j], sgf[q]}; },
+      [&](const G2 &v) {
+        const float3 gw = grad_from(v.s, sub3(v.a, p));
+        tr1 += dot3(make_float3(va.x - v.b.x, va.y - v.b.y, va.z - v.b.z), gw);
+        dx1 = dx1 - gw.x;
+        dy1 = dy1 - gw.y;
+        dz1 = dz1 - gw.z;
+      });
+  for_neighbors<kBatch>(
+      b.nbr_fs, n, i, ns, [&](int j, long q) { return G2{bpos[j], bvel[j], sgs[q]}; },
+      [&](const G2 &v) {
+        const float4 o = v.a, vb = v.b;
+        const float3 gw = grad_from(v.s, sub3(o, p));
+        tr2 += dot3(make_float3(va.x - vb.x, va.y - vb.y, va.z - vb.z), gw) * o.w;
+        dx2 = dx2 - gw.x * o.w;
+        dy2 = dy2 - gw.y * o.w;
+        dz2 = dz2 - gw.z * o.w;
+      });
   const float rho_adv = p.w + (tr1 * c.mass + tr2) * dt;
   const float dix = (dx1 * c.mass + dx2) * dt2 / ri2;
   const float diy = (dy1 * c.mass + dy2) * dt2 / ri2;
   const float diz = (dz1 * c.mass + dz2) * dt2 / ri2;
   float ta1 = 0.0f, ta2 = 0.0f;
-  for (int k = 0; k < nf; ++k) {
-    const float4 o = b.pos[b.nbr_ff[(long)k * n + i]];
-    const float3 gw = GW_OF(T, sub3(o, p));
-    const float3 dd = make_float3(dix - gw.x * t3 / ri2, diy - gw.y * t3 / ri2,
-                                  diz - gw.z * t3 / ri2);
-    ta1 += dot3(dd, gw);
-  }
-  for (int k = 0; k < ns; ++k) {
-    const float4 o = b.bpos[b.nbr_fs[(long)k * n + i]];
-    const float3 gw = GW_OF(T, sub3(o, p));
-    ta2 += dot3(make_float3(dix, diy, diz), gw) * o.w;
-  }
+  for_neighbors<kBatch>(
+      b.nbr_ff, n, i, nf, [&](int j, long q) { return G1{pos[j], sgf[q]}; },
+      [&](const G1 &v) {
+        const float3 gw = grad_from(v.s, sub3(v.a, p));
+        const float3 dd = make_float3(dix - div_by(gw.x * t3, ri2, yi),
+                                      diy - div_by(gw.y * t3, ri2, yi),
+                                      diz - div_by(gw.z * t3, ri2, yi));
+        ta1 += dot3(dd, gw);
+      });
+  for_neighbors<kBatch>(
+      b.nbr_fs, n, i, ns, [&](int j, long q) { return G1{bpos[j], sgs[q]}; },
+      [&](const G1 &v) {
+        const float3 gw = grad_from(v.s, sub3(v.a, p));
+        ta2 += dot3(make_float3(dix, diy, diz), gw) * v.a.w;
+      });
   const float a_ii = ta1 * c.mass + ta2;
   b.aux[i] = make_float4(rho_adv, a_ii, 0.0f, 0.0f);
   float pv;
@@ -286,30 +367,35 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
 }
 
 // ---- compute_term_Ap_1, IISPH.py:431-466 ---------------------------------
+
 template <typename TT>
 __device__ __forceinline__ void j1_particle(const Bufs &b, const SphConst &c, const TT &T, int i,
                                             int src, float t3) {
   const int n = b.n;
   const float4 p = b.pos[i];
   const float4 *__restrict__ dp = b.dp[src];
+  const float4 *__restrict__ pos = b.pos;
   const float pi = dp[i].w;
   const float ri2 = p.w * p.w;
-  const int nf = b.cnt_ff[i];
+  const float yi = 1.0f / ri2;
   float t1 = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  for (int k = 0; k < nf; ++k) {
-    const int j = b.nbr_ff[(long)k * n + i];
-    const float4 o = b.pos[j];
-    const float4 dj = dp[j];
-    const float3 gw = GW_OF(T, sub3(o, p));
-    const float3 term = make_float3(gw.x * t3 / ri2 * pi - dj.x * dj.w,
-                                    gw.y * t3 / ri2 * pi - dj.y * dj.w,
-                                    gw.z * t3 / ri2 * pi - dj.z * dj.w);
-    t1 += dot3(term, gw);
-    const float rj2 = o.w * o.w;
-    sx = sx + -gw.x * t3 / rj2 * dj.w;
-    sy = sy + -gw.y * t3 / rj2 * dj.w;
-    sz = sz + -gw.z * t3 / rj2 * dj.w;
-  }
+  const float *__restrict__ sgf = b.sg_ff;
+  for_neighbors<kBatch>(
+      b.nbr_ff, n, i, b.cnt_ff[i], [&](int j, long q) { return G2{pos[j], dp[j], sgf[q]}; },
+      [&](const G2 &v) {
+        const float4 o = v.a, dj = v.b;
+        const float3 gw = grad_from(v.s, sub3(o, p));
+        // d_ji = grad W_ij * t3 / rho_i^2, d_ij = -grad W_ij * t3 / rho_j^2 (IISPH.py:372-373)
+        const float3 term = make_float3(div_by(gw.x * t3, ri2, yi) * pi - dj.x * dj.w,
+                                        div_by(gw.y * t3, ri2, yi) * pi - dj.y * dj.w,
+                                        div_by(gw.z * t3, ri2, yi) * pi - dj.z * dj.w);
+        t1 += dot3(term, gw);
+        const float rj2 = o.w * o.w;
+        const float yj = 1.0f / rj2;
+        sx = sx + div_by(-gw.x * t3, rj2, yj) * dj.w;
+        sy = sy + div_by(-gw.y * t3, rj2, yj) * dj.w;
+        sz = sz + div_by(-gw.z * t3, rj2, yj) * dj.w;
+      });
   b.sap[i] = make_float4(sx, sy, sz, t1 * c.mass);
 }
 
@@ -321,20 +407,24 @@ __device__ __forceinline__ void j2_particle(const Bufs &b, const SphConst &c, co
   const int n = b.n;
   const float4 p = b.pos[i];
   const float4 si = b.sap[i];
-  const int nf = b.cnt_ff[i], ns = b.cnt_fs[i];
+  const float4 *__restrict__ pos = b.pos;
+  const float4 *__restrict__ sap = b.sap;
+  const float4 *__restrict__ bpos = b.bpos;
   float t1 = 0.0f, t2 = 0.0f;
-  for (int k = 0; k < nf; ++k) {
-    const int j = b.nbr_ff[(long)k * n + i];
-    const float4 o = b.pos[j];
-    const float4 sj = b.sap[j];
-    const float3 gw = GW_OF(T, sub3(o, p));
-    t1 += dot3(make_float3(si.x - sj.x, si.y - sj.y, si.z - sj.z), gw);
-  }
-  for (int k = 0; k < ns; ++k) {
-    const float4 o = b.bpos[b.nbr_fs[(long)k * n + i]];
-    const float3 gw = GW_OF(T, sub3(o, p));
-    t2 += dot3(make_float3(si.x, si.y, si.z), gw) * o.w;
-  }
+  const float *__restrict__ sgf = b.sg_ff;
+  const float *__restrict__ sgs = b.sg_fs;
+  for_neighbors<kBatch>(
+      b.nbr_ff, n, i, b.cnt_ff[i], [&](int j, long q) { return G2{pos[j], sap[j], sgf[q]}; },
+      [&](const G2 &v) {
+        const float3 gw = grad_from(v.s, sub3(v.a, p));
+        t1 += dot3(make_float3(si.x - v.b.x, si.y - v.b.y, si.z - v.b.z), gw);
+      });
+  for_neighbors<kBatch>(
+      b.nbr_fs, n, i, b.cnt_fs[i], [&](int j, long q) { return G1{bpos[j], sgs[q]}; },
+      [&](const G1 &v) {
+        const float3 gw = grad_from(v.s, sub3(v.a, p));
+        t2 += dot3(make_float3(si.x, si.y, si.z), gw) * v.a.w;
+      });
   const float4 di = b.dp[src][i];
   const float4 au = b.aux[i];
   const float ap = si.w + (t1 * c.mass + t2 + au.y * di.w);
@@ -376,7 +466,7 @@ __device__ __forceinline__ void block_residual_add(DevState *st, int slot,
 
 template <bool kLds>
 __global__ void k_j1(Bufs b, SphConst c, int src) {
-  const Tabs T = stage_tabs<kLds>(b, c);
+  const Tabs T{b.wt, b.gt};  // grad W comes from the stored pair values
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const float dt = (float)b.st->dt;
   const float t3 = c.mass * (dt * dt);
@@ -385,7 +475,7 @@ __global__ void k_j1(Bufs b, SphConst c, int src) {
 
 template <bool kLds>
 __global__ void k_j2(Bufs b, SphConst c, int src, int slot) {
-  const Tabs T = stage_tabs<kLds>(b, c);
+  const Tabs T{b.wt, b.gt};  // grad W comes from the stored pair values
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long rs = 0;
   unsigned rc = 0;
@@ -393,8 +483,78 @@ __global__ void k_j2(Bufs b, SphConst c, int src, int slot) {
   if (slot >= 0) block_residual_add(b.st, slot, rs, rc);
 }
 
+// ---- device-resident Jacobi loop control ----------------------------------
+// Each iteration is J1, J2 and a one-thread check; the kernels read the
+// iteration counter and the done flag from DevState and return at once when
+// the loop has finished, so the host can enqueue iterations ahead without
+// knowing when the loop stops.
+template <bool kLds>
+__global__ void k_j1_dev(Bufs b, SphConst c, int src0) {
+  if (__hip_atomic_load(&b.st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const Tabs T{b.wt, b.gt};  // grad W comes from the stored pair values
+  const int k = b.st->iter;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const float dt = (float)b.st->dt;
+  const float t3 = c.mass * (dt * dt);
+  if (i < b.n) j1_particle(b, c, T, i, src0 ^ (k & 1), t3);
+}
+
+// Stop test of iteration k (IISPH.py:1200-1207) on the residual of J2.
+__device__ __forceinline__ void jacobi_check(DevState *st, int k, int src0, int min_it,
+                                             int max_it, int force_it, double eta) {
+  const unsigned long long S =
+      __hip_atomic_load(&st->res_sum[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned Cn = __hip_atomic_load(&st->res_cnt[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  double e;
+  const int cont = jacobi_continue(k, S, Cn, min_it, max_it, force_it, eta, &e);
+  if (cont) {
+    st->iter = k + 1;
+  } else {
+    st->done = 1;
+    st->iterations = k + 1;
+    st->ap_evals += k + 1;
+    st->residual = e;
+    st->converged = e <= eta;
+    st->final_src = src0 ^ (k & 1);
+    st->done_tag = st->step_tag;
+  }
+}
+
+// J2 of a device-driven iteration; the last block to finish runs the stop test.
+template <bool kLds>
+__global__ void k_j2_dev(Bufs b, SphConst c, int src0, int min_it, int max_it, int force_it,
+                         double eta) {
+  __shared__ int s_last;
+  if (__hip_atomic_load(&b.st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const Tabs T{b.wt, b.gt};  // grad W comes from the stored pair values
+  const int k = b.st->iter;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long rs = 0;
+  unsigned rc = 0;
+  if (i < b.n) j2_particle(b, c, T, i, src0 ^ (k & 1), rs, rc);
+  block_residual_add(b.st, k, rs, rc);
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned t = __hip_atomic_fetch_add(&b.st->blocks_done, 1u, __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    s_last = t == gridDim.x - 1;
+    if (s_last) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      b.st->blocks_done = 0u;
+      jacobi_check(b.st, k, src0, min_it, max_it, force_it, eta);
+    }
+  }
+}
+
 // ---- kick (IISPH.py:674-688) and drift (IISPH.py:691-699) ----------------
+// Also keeps the final pressure (pfinal) and the pressure carried to the
+// next step's start value (pprev): zero where a_ii fell inside the guard band
+// of update_p (|a_ii| < INV_SMALL, IISPH.py:535), because the pressure of such
+// a weakly coupled particle grows without bound and means nothing.
 __global__ void k_integrate(Bufs b, SphConst c, int src) {
+  __shared__ unsigned s_vmax[16];
+  __shared__ int s_bad[16];
+  if (src < 0) src = b.st->final_src;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned vbits = 0;
   int bad = 0;
@@ -414,15 +574,44 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
     const float x = p.x + vx * dt, y = p.y + vy * dt, z = p.z + vz * dt;
     b.pos[i] = make_float4(x, y, z, p.w);
     b.vel[i] = make_float4(vx, vy, vz, 0.0f);
-    b.pprev[i] = di.w;
+    b.pfinal[i] = di.w;
+    const float a = b.aux[i].y;
+    b.pprev[i] = (a > -c.inv_small && a < c.inv_small) ? 0.0f : di.w;
     bad = !(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(l));
     if (bad) vbits = 0;
   }
   vbits = wave_max_u32(vbits);
   bad = wave_max_i32(bad);
-  if ((threadIdx.x & 63) == 0) {
-    if (vbits) atomicMax(&b.st->vmax_bits, vbits);
-    if (bad) atomicOr(&b.st->status, (int)ST_NAN);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_vmax[wid] = vbits;
+    s_bad[wid] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned m = 0;
+    int bb = 0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
+      m = max(m, s_vmax[w]);
+      bb |= s_bad[w];
+    }
+    b.vmax_part[blockIdx.x] = m;
+    if (bb) atomicOr(&b.st->status, (int)ST_NAN);
+  }
+}
+
+// max |v| over the per-block partial maxima of k_integrate
+__global__ void k_vmax_reduce(Bufs b, int nparts) {
+  unsigned m = 0;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) m = max(m, b.vmax_part[k]);
+  m = wave_max_u32(m);
+  __shared__ unsigned s[16];
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w) m = max(m, s[w]);
+    m = max(m, s[0]);
+    b.st->vmax_bits = m;
   }
 }
 
@@ -453,9 +642,14 @@ __global__ void k_reset_state(DevState *st) {
     st->max_fs = 0;
     st->status = 0;
     st->done = 0;
+    st->step_tag += 1;
+    st->blocks_done = 0u;
+    st->iter = 0;
+    st->final_src = 0;
     st->residual = 0.0;
     st->bar_count = 0u;
     st->bar_gen = 0u;
+    st->bar_seen = 0u;
   }
 }
 
@@ -463,19 +657,20 @@ __global__ void k_reset_state(DevState *st) {
 // Grid-wide barrier with a bounded wait: a wait that exceeds the spin limit
 // raises ST_BARRIER and every block leaves the loop, so the kernel always
 // ends.
-constexpr long kBarrierSpinLimit = 1L << 24;
+constexpr long kBarrierSpinLimit = 1L << 19;
 
 __device__ __forceinline__ bool grid_barrier(DevState *st) {
   __shared__ int s_abort;
-  __threadfence();
+  // every wave's stores have completed at the workgroup barrier; one thread
+  // per block then publishes them at agent scope and waits
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned nblk = gridDim.x;
     const unsigned gen =
         __hip_atomic_load(&st->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned arrived =
         __hip_atomic_fetch_add(&st->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    int abort = 0;
     if (arrived == nblk - 1) {
       __hip_atomic_store(&st->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&st->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -483,25 +678,30 @@ __device__ __forceinline__ bool grid_barrier(DevState *st) {
       long spins = 0;
       while (__hip_atomic_load(&st->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
         __builtin_amdgcn_s_sleep(2);
-        if (++spins > kBarrierSpinLimit) {
+        if ((++spins & 255) == 0 &&
+            (__hip_atomic_load(&st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+             ST_BARRIER))
+          break;
+        if (spins > kBarrierSpinLimit) {
+          atomicMax(&st->bar_seen,
+                    __hip_atomic_load(&st->bar_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
           atomicOr(&st->status, (int)ST_BARRIER);
           break;
         }
       }
     }
-    abort = __hip_atomic_load(&st->status, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) &
-            ST_BARRIER;
-    s_abort = abort;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    s_abort = __hip_atomic_load(&st->status, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) &
+              ST_BARRIER;
   }
   __syncthreads();
-  __threadfence();
   return s_abort == 0;
 }
 
 template <bool kLds>
 __global__ void k_jacobi_persistent(Bufs b, SphConst c, int src, int min_it, int max_it,
                                     int force_it, double eta) {
-  const Tabs T = stage_tabs<kLds>(b, c);
+  const Tabs T{b.wt, b.gt};  // grad W comes from the stored pair values
   const float dt = (float)b.st->dt;
   const float t3 = c.mass * (dt * dt);
   const int stride = gridDim.x * blockDim.x;
@@ -566,7 +766,7 @@ void launch_boundary_psi(const Bufs &b, const SphConst &c, hipStream_t s) {
 
 void launch_lists(const Bufs &b, const SphConst &c, int block, hipStream_t s) {
   if (b.n == 0) return;
-  k_lists<<<nblocks(b.n, block), block, 0, s>>>(b, c);
+  DISPATCH_LDS(k_lists, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c);
 }
 
 void launch_density(const Bufs &b, const SphConst &c, bool fuse_vadv, int block, hipStream_t s) {
@@ -594,9 +794,20 @@ void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, h
   DISPATCH_LDS(k_j2, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src, slot);
 }
 
+void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P, int src0,
+                         int iters, int block, hipStream_t s) {
+  if (b.n == 0) return;
+  for (int k = 0; k < iters; ++k) {
+    DISPATCH_LDS(k_j1_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0);
+    DISPATCH_LDS(k_j2_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0,
+                 P.min_iters, P.max_iters, P.force_iters, P.eta);
+  }
+}
+
 void launch_integrate(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s) {
   if (b.n == 0) return;
   k_integrate<<<nblocks(b.n, block), block, 0, s>>>(b, c, src);
+  k_vmax_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block));
 }
 
 void launch_clock(const Bufs &b, const wsph_params &P, hipStream_t s) {

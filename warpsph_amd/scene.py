@@ -17,6 +17,11 @@ import numpy as np
 from .config import SolverConfig
 
 
+def _c32(a: np.ndarray) -> np.ndarray:
+    """C-contiguous float32 copy (device handoff reads raw row-major memory)."""
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
 @dataclass
 class Scene:
     x: np.ndarray            # (n, 3) float32 fluid positions
@@ -68,8 +73,8 @@ def reference_box(cfg: SolverConfig) -> Scene:
     axes = [np.arange(min_point[a], max_point[a], cfg.spacing) for a in range(3)]
     xx, yy, zz = np.meshgrid(*axes, indexing="ij")
     x = np.stack([xx.ravel(), yy.ravel(), zz.ravel()]).T
-    x32 = x.astype(np.float32)
-    xb32 = xb.astype(np.float32)
+    x32 = _c32(x)
+    xb32 = _c32(xb)
     lo, hi = _bounds(x32, xb32)
     return Scene(x32, xb32, np.zeros_like(xb32), xb32.shape[0], lo, hi)
 
@@ -94,8 +99,8 @@ def dam_break(cfg: SolverConfig) -> Scene:
     wall layers."""
     xb = box_shell(cfg.box, cfg.spacing, cfg.boundary_layers)
     x = lattice_block(cfg.fluid_cells, cfg.spacing, cfg.fluid_origin, cfg.jitter, cfg.seed)
-    x32 = x.astype(np.float32)
-    xb32 = xb.astype(np.float32)
+    x32 = _c32(x)
+    xb32 = _c32(xb)
     lo, hi = _bounds(x32, xb32)
     return Scene(x32, xb32, np.zeros_like(xb32), xb32.shape[0], lo, hi)
 

@@ -41,9 +41,11 @@ class IISPH:
 
     def __init__(self, stage_path: Optional[str] = "example_sph.usd", preview: bool = False,
                  verbose: bool = False, config: Optional[cfgmod.SolverConfig] = None,
-                 device: Optional[str] = None, scene: Optional[scn.Scene] = None):
+                 device: Optional[str] = None, scene: Optional[scn.Scene] = None,
+                 collect_stats: bool = False):
         torch = _torch()
         self.verbose = verbose
+        self.collect_stats = collect_stats
         self.cfg = cfg = config if config is not None else cfgmod.reference()
         self.device = torch.device(device or "cuda:0")
         if self.device.type != "cuda" or not torch.cuda.is_available():
@@ -74,12 +76,12 @@ class IISPH:
         N.check("wsph_set_tables", self._lib.wsph_set_tables(
             h, self.W_table.ctypes.data, self.W_table.shape[0],
             self.grad_W_table.ctypes.data, self.grad_W_table.shape[0]))
-        self.boundary_x = torch.from_numpy(sc.xb).to(self.device)
-        self.boundary_v = torch.from_numpy(sc.vb).to(self.device)
+        self.boundary_x = torch.from_numpy(np.ascontiguousarray(sc.xb, np.float32)).to(self.device)
+        self.boundary_v = torch.from_numpy(np.ascontiguousarray(sc.vb, np.float32)).to(self.device)
         _sync_torch()
         N.check("wsph_set_boundary", self._lib.wsph_set_boundary(
             h, self.boundary_x.data_ptr(), self.boundary_v.data_ptr(), self.boundary_n))
-        x0 = torch.from_numpy(sc.x).to(self.device)
+        x0 = torch.from_numpy(np.ascontiguousarray(sc.x, np.float32)).to(self.device)
         _sync_torch()
         N.check("wsph_set_fluid", self._lib.wsph_set_fluid(h, x0.data_ptr(), None, self.n))
         self.last_stats: Optional[dict] = None
@@ -113,7 +115,8 @@ class IISPH:
 
     # ---- the step (IISPH.py:1002-1371) ----------------------------------
     def _need_stats(self) -> bool:
-        return (self.verbose or self.cfg.dt_mode == cfgmod.DT_CFL or self.renderer is not None)
+        return (self.collect_stats or self.verbose or self.cfg.dt_mode == cfgmod.DT_CFL
+                or self.renderer is not None)
 
     def step(self):
         """Advance one time step (IISPH.step, IISPH.py:1002)."""
@@ -149,7 +152,7 @@ class IISPH:
         out = (C.c_longlong * 8)()
         N.check("wsph_get_counters", self._lib.wsph_get_counters(self._h, out, 8))
         return dict(steps=out[0], iterations=out[1], nonconverged=out[2], ap_evals=out[3],
-                    status=out[4])
+                    status=out[4], barrier_seen=out[5], persistent_grid=out[6])
 
     def set_pressure(self, p) -> None:
         """Caller-supplied start pressure for the next step (original order)."""
