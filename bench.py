@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--warm-start", choices=["zero", "previous"], default="previous")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--layout", type=int, default=-1, help="cell layout: 0 linear, 1 Morton")
     ap.add_argument("--coop-per-cu", type=int, default=0,
                     help="persistent kernel blocks per CU (0: solver default)")
     ap.add_argument("--trace-steps", type=int, default=0,
@@ -132,6 +133,8 @@ def main():
                       chunk_iters=args.chunk,
                       warm_start=config.WARM_ZERO if args.warm_start == "zero"
                       else config.WARM_PREVIOUS)
+    if args.layout >= 0:
+        cfg = cfg.replace(cell_layout=args.layout)
     if args.coop_per_cu:
         cfg = cfg.replace(coop_blocks_per_cu=args.coop_per_cu)
     t_setup = time.perf_counter()
@@ -204,7 +207,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(workload),
-                     "kernel": "k_jacobi_persistent",
+                     "kernel": "jacobi (k_j1_dev + k_j2_dev per A p evaluation)",
                      "algorithmic_bytes_per_launch": jac_bytes,
                      "avg_launch_ms": jac_ms},
         "step_roofline_frac": (n * (BYTES_PER_STEP_FIXED * steps + BYTES_PER_AP_EVAL * ap_evals)

@@ -129,6 +129,11 @@ typedef struct wsph_solver wsph_solver;
 const char *wsph_last_error(void);
 int wsph_abi_version(void);
 
+/* Neighbour predicate used by the grid queries: for float32 d2 >= 0,
+ * sqrtf(d2) < h (the test of IISPH.py:72 and :129) holds exactly when
+ * d2 < wsph_radius_sq_threshold(h). */
+float wsph_radius_sq_threshold(float h);
+
 /* Create a solver (replaces IISPH.__init__'s allocations, IISPH.py:770-805). */
 int wsph_create(const wsph_params *params, wsph_solver **out);
 /* Release every device buffer the solver allocated. */

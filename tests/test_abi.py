@@ -46,3 +46,18 @@ def test_create_rejects_bad_params_without_gpu_work():
     assert b"iteration" in lib.wsph_last_error()
     p = N.make_params(config.dambreak_8k(), (0, 6, 6), (6, 6, 6))
     assert lib.wsph_create(ctypes.byref(p), ctypes.byref(h)) == -1
+
+
+@pytest.mark.parametrize("h", [0.04, 0.02, 0.01, 0.1, 0.0371])
+def test_radius_threshold_matches_sqrt_predicate(h):
+    """d2 < threshold exactly when float32 sqrt(d2) < h, checked on every
+    float32 within 2^16 ulps of the threshold and on random radicands."""
+    import numpy as np
+    lib = N.load()
+    t = np.float32(lib.wsph_radius_sq_threshold(h))
+    hf = np.float32(h)
+    bits = t.view(np.uint32)
+    near = (np.arange(-65536, 65536, dtype=np.int64) + int(bits)).astype(np.uint32).view(np.float32)
+    rnd = np.random.default_rng(1).uniform(0, 4 * float(t), 200000).astype(np.float32)
+    for d2 in (near, rnd):
+        assert np.array_equal(np.sqrt(d2) < hf, d2 < t)

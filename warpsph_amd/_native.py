@@ -91,6 +91,7 @@ _I = C.c_int
 SIGNATURES = [
     ("wsph_last_error", C.c_char_p, []),
     ("wsph_abi_version", _I, []),
+    ("wsph_radius_sq_threshold", C.c_float, [C.c_float]),
     ("wsph_create", _I, [C.POINTER(WsphParams), C.POINTER(_P)]),
     ("wsph_destroy", _I, [_P]),
     ("wsph_set_tables", _I, [_P, _P, _I, _P, _I]),
@@ -128,7 +129,7 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = Path(path) if path else LIB_PATH
+    p = Path(path) if path else Path(os.environ.get("WSPH_LIB", LIB_PATH))
     if not p.exists():
         raise RuntimeError(
             f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "

@@ -36,29 +36,33 @@ def _hipcc() -> str:
     return "hipcc"
 
 
-def _compile(src: str) -> Path:
-    obj = OBJ_DIR / (Path(src).stem + ".o")
-    cmd = [_hipcc(), *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+def _compile(src: str, extra=(), obj_dir=None) -> Path:
+    obj = (obj_dir or OBJ_DIR) / (Path(src).stem + ".o")
+    cmd = [_hipcc(), *FLAGS, *extra, "-c", str(CSRC / src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
     return obj
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every HIP source and link libwsph.so (always rebuilds)."""
-    OBJ_DIR.mkdir(exist_ok=True)
+def build(verbose: bool = False, out: Path = OUT, defines=()) -> Path:
+    """Compile every HIP source and link libwsph.so (always rebuilds).
+    ``defines`` are extra -D flags (tuning constants of the kernels)."""
+    extra = [f"-D{d}" for d in defines]
+    obj_dir = OBJ_DIR / ("_".join(defines) or "default")
+    obj_dir.mkdir(parents=True, exist_ok=True)
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = OUT.with_suffix(".so.tmp")
+        objs = list(ex.map(lambda src: _compile(src, extra, obj_dir), SOURCES))
+    out = Path(out)
+    tmp = out.with_suffix(".so.tmp")
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, OUT)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {OUT}")
-    return OUT
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":

@@ -141,10 +141,11 @@ struct wsph_solver {
   float4 *vadv = nullptr, *dp[2] = {nullptr, nullptr}, *sap = nullptr, *aux = nullptr;
   float *ap = nullptr, *pstart = nullptr, *p_ext = nullptr, *pfinal = nullptr;
   unsigned *vmax_part = nullptr;
+  unsigned long long *res_part = nullptr;
+  unsigned *cnt_part = nullptr;
   bool have_p_ext = false;
   int *slot = nullptr, *rank = nullptr, *tmp = nullptr, *perm = nullptr;
   int *nbr_ff = nullptr, *cnt_ff = nullptr, *nbr_fs = nullptr, *cnt_fs = nullptr;
-  float *sg_ff = nullptr, *sg_fs = nullptr;
   GridBufs fg, bg;
   // boundary
   float4 *bpos = nullptr, *bvel = nullptr;
@@ -164,6 +165,24 @@ struct wsph_solver {
   hipEvent_t chunk_ev[2] = {};
   int step_count = 0;
 };
+
+// Smallest float32 d2 >= 0 with sqrtf(d2) >= h: then sqrtf(d2) < h exactly
+// when d2 < threshold, because rounded sqrt is monotone.
+static float radius_sq_threshold(float h) {
+  uint32_t lo = 0, hi = 0x7f7fffffu;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    float s;
+    memcpy(&s, &mid, sizeof(s));
+    if (sqrtf(s) >= h)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  float r;
+  memcpy(&r, &lo, sizeof(r));
+  return r;
+}
 
 static long grid_cells(const int *dims, int layout) {
   if (layout == WSPH_LAYOUT_MORTON) return (long)morton3(dims[0] - 1, dims[1] - 1, dims[2] - 1) + 1;
@@ -209,14 +228,14 @@ static Bufs make_bufs(wsph_solver *s) {
   b.pstart = s->pstart;
   b.pfinal = s->pfinal;
   b.vmax_part = s->vmax_part;
+  b.res_part = s->res_part;
+  b.cnt_part = s->cnt_part;
   b.fstart = s->fg.start;
   b.fcount = s->fg.count;
   b.nbr_ff = s->nbr_ff;
   b.cnt_ff = s->cnt_ff;
   b.nbr_fs = s->nbr_fs;
   b.cnt_fs = s->cnt_fs;
-  b.sg_ff = s->sg_ff;
-  b.sg_fs = s->sg_fs;
   b.cap_ff = s->P.max_ff;
   b.cap_fs = s->P.max_fs;
   b.bpos = s->bpos;
@@ -239,6 +258,7 @@ static int check_dims(const int *d, const char *what) {
 extern "C" {
 
 const char *wsph_last_error(void) { return g_err; }
+float wsph_radius_sq_threshold(float h) { return radius_sq_threshold(h); }
 int wsph_abi_version(void) { return WSPH_ABI_VERSION; }
 
 int wsph_create(const wsph_params *params, wsph_solver **out) {
@@ -249,7 +269,8 @@ int wsph_create(const wsph_params *params, wsph_solver **out) {
   if (P.max_iters < 1 || P.max_iters > kMaxIters || P.force_iters < 0 ||
       P.force_iters > kMaxIters || P.min_iters < 0)
     return fail(WSPH_E_ARG, "iteration bounds must satisfy 1 <= max_iters <= %d", kMaxIters);
-  if (P.max_ff < 1 || P.max_fs < 1) return fail(WSPH_E_ARG, "neighbour capacities must be >= 1");
+  if (P.max_ff < 4 || P.max_fs < 4 || (P.max_ff & 3) || (P.max_fs & 3))
+    return fail(WSPH_E_ARG, "neighbour capacities must be positive multiples of 4");
   if (P.block < 64 || P.block > 1024 || (P.block & 63))
     return fail(WSPH_E_ARG, "block must be a multiple of 64 in [64, 1024]");
   if (P.cell_layout != WSPH_LAYOUT_LINEAR && P.cell_layout != WSPH_LAYOUT_MORTON)
@@ -264,6 +285,7 @@ int wsph_create(const wsph_params *params, wsph_solver **out) {
   SphConst &C = s->C;
   memset(&C, 0, sizeof(C));
   C.h = (float)P.h;
+  C.r2_lt = radius_sq_threshold(C.h);
   C.cell_inv = 1.0f / C.h;
   C.mass = (float)P.mass;
   C.rho0 = (float)P.rho0;
@@ -325,6 +347,8 @@ static void free_fluid(wsph_solver *s) {
   dfree(s->p_ext);
   dfree(s->pfinal);
   dfree(s->vmax_part);
+  dfree(s->res_part);
+  dfree(s->cnt_part);
   dfree(s->slot);
   dfree(s->rank);
   dfree(s->tmp);
@@ -333,8 +357,6 @@ static void free_fluid(wsph_solver *s) {
   dfree(s->cnt_ff);
   dfree(s->nbr_fs);
   dfree(s->cnt_fs);
-  dfree(s->sg_ff);
-  dfree(s->sg_fs);
   s->n = 0;
 }
 
@@ -474,6 +496,8 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
     HIP_TRY(dalloc(&s->pstart, n));
     HIP_TRY(dalloc(&s->pfinal, n));
     HIP_TRY(dalloc(&s->vmax_part, (size_t)(n + 63) / 64 + 1));
+    HIP_TRY(dalloc(&s->res_part, (size_t)(n + 63) / 64 + 1));
+    HIP_TRY(dalloc(&s->cnt_part, (size_t)(n + 63) / 64 + 1));
     HIP_TRY(dalloc(&s->p_ext, n));
     HIP_TRY(dalloc(&s->slot, n));
     HIP_TRY(dalloc(&s->rank, n));
@@ -483,8 +507,6 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
     HIP_TRY(dalloc(&s->cnt_fs, n));
     HIP_TRY(dalloc(&s->nbr_ff, (size_t)n * s->P.max_ff));
     HIP_TRY(dalloc(&s->nbr_fs, (size_t)n * s->P.max_fs));
-    HIP_TRY(dalloc(&s->sg_ff, (size_t)n * s->P.max_ff));
-    HIP_TRY(dalloc(&s->sg_fs, (size_t)n * s->P.max_fs));
     s->cap_n = n;
   }
   s->n = n;

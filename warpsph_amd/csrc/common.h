@@ -28,6 +28,7 @@ enum : int {
 // literals the reference's kernels see, wp_sph/main.py:25-61).
 struct SphConst {
   float h;        // SMOOTHING_LENGTH
+  float r2_lt;    // sqrtf(d2) < h  <=>  d2 < r2_lt (sqrt is monotone, rounded to nearest)
   float cell_inv; // 1.0f / h, hash grid cell width inverse
   float w_scale;  // float(W_SIZE - 1) * h_inv   (sph_funcs.py:98)
   float g_scale;  // float(GRAD_W_SIZE - 1) * h_inv (sph_funcs.py:113)
@@ -53,7 +54,6 @@ struct DevState {
   int final_src;              // dp buffer holding the final iterate
   int step_tag;               // incremented by every step's state reset
   int done_tag;               // step_tag of the step whose Jacobi loop finished last
-  unsigned blocks_done;       // blocks of the running J2 that have finished
   unsigned bar_count, bar_gen;  // grid barrier
   unsigned bar_seen;            // arrivals seen by a barrier wait that timed out
   unsigned long long res_sum[kMaxIters];
@@ -158,11 +158,30 @@ __device__ __forceinline__ float div_by(float a, float b, float y) {
   return __builtin_fmaf(r, y, q0);
 }
 
-// Visit the neighbours of particle i from a column-major list in order,
-// loading kBatch indices and their gathers before evaluating them, so that
-// several memory round trips are in flight per thread. `load(j, slot)`
-// returns the data of neighbour j (slot = its list position k * n + i, for
-// per-pair arrays), `body(data)` accumulates in list order.
+// Workgroups are dispatched round-robin over the 8 XCDs of the device, each
+// with its own L2. Map the hardware block index to a logical tile so that
+// every XCD works on one contiguous range of (cell-sorted) particles and the
+// neighbour data it gathers stays in its own L2.
+#ifndef WSPH_XCD_REMAP
+#define WSPH_XCD_REMAP 1
+#endif
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile() {
+  if (!WSPH_XCD_REMAP) return blockIdx.x;
+  const int G = gridDim.x, b = blockIdx.x;
+  const int q = G / kXcds, r = G % kXcds;
+  const int x = b % kXcds, k = b / kXcds;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+__device__ __forceinline__ int particle_index() { return xcd_tile() * blockDim.x + threadIdx.x; }
+
+// Visit the neighbours of particle i in list order. Lists are column-major,
+// entry k of particle i at list[k * n + i], so a wave reads slot k of 64
+// consecutive particles in one coalesced access. kBatch indices and their
+// gathers are issued before the batch is evaluated, so several memory round
+// trips are in flight per thread. `load(j, slot)` returns the data of
+// neighbour j (slot = its list position), `body(data)` accumulates in list
+// order.
 template <int kBatch, typename LoadF, typename BodyF>
 __device__ __forceinline__ void for_neighbors(const int *__restrict__ list, int n, int i, int cnt,
                                               LoadF load, BodyF body) {

@@ -44,12 +44,13 @@ struct Bufs {
   float *pprev;     // pressure carried to the next step's start value
   float *pfinal;    // final pressure of the last step
   unsigned *vmax_part;  // per-block max |v| of k_integrate
+  unsigned long long *res_part;  // per-block residual sums of J2
+  unsigned *cnt_part;            // per-block residual counts of J2
   const float *pstart;  // caller-supplied start pressure, sorted order (or null)
   // fluid grid
   const unsigned *fstart, *fcount;
   // neighbour lists, column-major: list[k * n + i]
   int *nbr_ff, *cnt_ff, *nbr_fs, *cnt_fs;
-  float *sg_ff, *sg_fs;  // per-pair grad-W table value, same layout as the lists
   int cap_ff, cap_fs;
   // boundary (sorted by its own grid)
   float4 *bpos;     // x, y, z, psi

@@ -29,38 +29,32 @@ __device__ __forceinline__ Tabs stage_tabs(const Bufs &b, const SphConst &c) {
   }
 }
 
-constexpr int kBatch = 4;  // neighbours loaded ahead per thread
+// Neighbours loaded ahead per thread, per pass (J1 is ALU-bound, the other
+// passes wait on memory).
+#ifndef WSPH_BATCH_J1
+#define WSPH_BATCH_J1 4
+#endif
+#ifndef WSPH_BATCH_J2
+#define WSPH_BATCH_J2 4
+#endif
+#ifndef WSPH_BATCH_OTHER
+#define WSPH_BATCH_OTHER 4
+#endif
+constexpr int kBatch = WSPH_BATCH_OTHER;
+constexpr int kBatchJ1 = WSPH_BATCH_J1;
+constexpr int kBatchJ2 = WSPH_BATCH_J2;
 
 struct F4x2 {
   float4 a, b;
 };
-struct G1 {  // neighbour data + the pair's grad-W table value
-  float4 a;
-  float s;
-};
-struct G2 {
-  float4 a, b;
-  float s;
-};
-
-// grad W_ij from the pair's stored table value s = G(|r|) (or 0), r = x_j - x_i:
-// the last step of grad_spline_W (sph_funcs.py:118)
-__device__ __forceinline__ float3 grad_from(float s, float3 r) {
-  return make_float3(s * r.x, s * r.y, s * r.z);
-}
-
-// table value of grad_spline_W at |r| = len (sph_funcs.py:105-117)
-__device__ __forceinline__ float grad_scalar(const float *G, int n, float scale, float h, float len) {
-  if (len >= h || len < 1e-10f) return 0.0f;
-  const float idx = len * scale;
-  int lo = (int)idx;
-  lo = lo < n - 2 ? lo : n - 2;
-  const float a = idx - (float)lo;
-  return (1.0f - a) * G[lo] + a * G[lo + 1];
-}
-
 #define W_OF(T, r) spline_w((T).w, c.nw, c.w_scale, c.h, (r))
 #define GW_OF(T, r) grad_w((T).g, c.ng, c.g_scale, c.h, (r))
+
+// (dx*dx + dy*dy) + dz*dz with d = a - b: the radicand of wp.length(a - b)
+__device__ __forceinline__ float dist2(float ax, float ay, float az, float bx, float by, float bz) {
+  const float dx = ax - bx, dy = ay - by, dz = az - bz;
+  return dx * dx + dy * dy + dz * dz;
+}
 
 __device__ __forceinline__ float3 sub3(float4 a, float4 b) {
   return make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
@@ -121,25 +115,17 @@ __device__ __forceinline__ void for_span(const float4 *__restrict__ pts, int s, 
   for (; j < e; ++j) test(j, pts[j]);
 }
 
-// Also stores, per pair, the grad-W table value at the pair distance: the
-// distance |x_i - x_j| of the neighbour test equals |x_j - x_i| of
-// grad_spline_W bit for bit, so later passes rebuild grad W_ij as s * r.
-template <bool kLds>
 __global__ void k_lists(Bufs b, SphConst c) {
-  const Tabs T = stage_tabs<kLds>(b, c);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = particle_index();
   int nf = 0, ns = 0;
   if (i < b.n) {
     const float4 p = b.pos[i];
     const int n = b.n;
     for_query_spans(p.x, p.y, p.z, c, c.fdims, b.fstart, b.fcount, [&](int s0, int e0) {
       for_span(b.pos, s0, e0, [&](int j, const float4 &o) {
-        const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
-        if (j != i && d < c.h) {
-          if (nf < b.cap_ff) {
-            b.nbr_ff[(long)nf * n + i] = j;
-            b.sg_ff[(long)nf * n + i] = grad_scalar(T.g, c.ng, c.g_scale, c.h, d);
-          }
+        // distance < SMOOTHING_LENGTH and index != i (IISPH.py:72)
+        if (j != i && dist2(p.x, p.y, p.z, o.x, o.y, o.z) < c.r2_lt) {
+          if (nf < b.cap_ff) b.nbr_ff[(long)nf * n + i] = j;
           ++nf;
         }
       });
@@ -147,12 +133,9 @@ __global__ void k_lists(Bufs b, SphConst c) {
     if (b.nb > 0)
       for_query_spans(p.x, p.y, p.z, c, c.bdims, b.bstart, b.bcount, [&](int s0, int e0) {
         for_span(b.bpos, s0, e0, [&](int j, const float4 &o) {
-          const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
-          if (d < c.h) {
-            if (ns < b.cap_fs) {
-              b.nbr_fs[(long)ns * n + i] = j;
-              b.sg_fs[(long)ns * n + i] = grad_scalar(T.g, c.ng, c.g_scale, c.h, d);
-            }
+          // distance < SMOOTHING_LENGTH (IISPH.py:129)
+          if (dist2(p.x, p.y, p.z, o.x, o.y, o.z) < c.r2_lt) {
+            if (ns < b.cap_fs) b.nbr_fs[(long)ns * n + i] = j;
             ++ns;
           }
         });
@@ -173,14 +156,14 @@ __global__ void k_lists(Bufs b, SphConst c) {
 template <bool kLds>
 __global__ void k_boundary_psi(Bufs b, SphConst c) {
   const Tabs T = stage_tabs<kLds>(b, c);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = particle_index();
   if (i >= b.nb) return;
   const float4 p = b.bpos[i];
   float rho = W_OF(T, 0.0f);
   for_query_spans(p.x, p.y, p.z, c, c.bdims, b.bstart, b.bcount, [&](int s0, int e0) {
     for_span(b.bpos, s0, e0, [&](int u, const float4 &o) {
-      const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
-      if (d < c.h && u != i) rho += W_OF(T, d);
+      const float d2 = dist2(p.x, p.y, p.z, o.x, o.y, o.z);
+      if (d2 < c.r2_lt && u != i) rho += W_OF(T, sqrtf(d2));
     });
   });
   reinterpret_cast<float *>(&b.bpos[i])[3] = c.rho0 / rho;
@@ -199,7 +182,7 @@ __device__ __forceinline__ float4 vadv_plain(float4 v, float rho, float dt, cons
 template <bool kLds>
 __global__ void k_density(Bufs b, SphConst c, int fuse_vadv) {
   const Tabs T = stage_tabs<kLds>(b, c);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = particle_index();
   if (i >= b.n) return;
   const int n = b.n;
   const float4 p = b.pos[i];
@@ -207,10 +190,10 @@ __global__ void k_density(Bufs b, SphConst c, int fuse_vadv) {
   const float4 *__restrict__ bpos = b.bpos;
   float t1 = 0.0f, t2 = 0.0f;
   for_neighbors<kBatch>(
-      b.nbr_ff, n, i, b.cnt_ff[i], [&](int j, long) { return pos[j]; },
+      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return pos[j]; },
       [&](const float4 &o) { t1 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z)); });
   for_neighbors<kBatch>(
-      b.nbr_fs, n, i, b.cnt_fs[i], [&](int j, long) { return bpos[j]; },
+      b.nbr_fs, b.n, i, b.cnt_fs[i], [&](int j, long) { return bpos[j]; },
       [&](const float4 &o) { t2 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z)) * o.w; });
   const float rho = t1 * c.mass + t2 + W_OF(T, 0.0f) * c.mass;
   reinterpret_cast<float *>(&b.pos[i])[3] = rho;
@@ -221,7 +204,7 @@ __global__ void k_density(Bufs b, SphConst c, int fuse_vadv) {
 template <bool kLds>
 __global__ void k_vadv(Bufs b, SphConst c) {
   const Tabs T = stage_tabs<kLds>(b, c);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = particle_index();
   if (i >= b.n) return;
   const int n = b.n;
   const float dt = (float)b.st->dt;
@@ -231,17 +214,15 @@ __global__ void k_vadv(Bufs b, SphConst c) {
   const float4 *__restrict__ pos = b.pos;
   const float4 *__restrict__ vel = b.vel;
   const float4 *__restrict__ bpos = b.bpos;
-  const float *__restrict__ sgf = b.sg_ff;
-  const float *__restrict__ sgs = b.sg_fs;
   float lx = 0.0f, ly = 0.0f, lz = 0.0f;
   if (c.vis_mu != 0.0f) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
     for_neighbors<kBatch>(
-        b.nbr_ff, n, i, nf, [&](int j, long q) { return G2{pos[j], vel[j], sgf[q]}; },
-        [&](const G2 &v) {
+        b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{pos[j], vel[j]}; },
+        [&](const F4x2 &v) {
           const float4 o = v.a, vj = v.b;
           const float3 xij = sub3(o, p);
-          const float3 gw = grad_from(v.s, xij);
+          const float3 gw = GW_OF(T, xij);
           const float3 vd = make_float3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z);
           const float dd = dot3(vd, xij);
           const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
@@ -251,11 +232,12 @@ __global__ void k_vadv(Bufs b, SphConst c) {
           az = az - gw.z * dd / o.w / r2;
         });
     for_neighbors<kBatch>(
-        b.nbr_fs, n, i, ns, [&](int j, long q) { return G1{bpos[j], sgs[q]}; },
-        [&](const G1 &v) {
+        b.nbr_fs, b.n, i, ns, [&](int j, long) { return bpos[j]; },
+        [&](const float4 &va_) {
+          const F4x2 v{va_, va_};
           const float4 o = v.a;
           const float3 xib = sub3(o, p);
-          const float3 gw = grad_from(v.s, xib);
+          const float3 gw = GW_OF(T, xib);
           const float dd = dot3(make_float3(vi.x, vi.y, vi.z), xib);
           const float d = dist3(p.x, p.y, p.z, o.x, o.y, o.z);
           const float r2 = d * d;
@@ -274,7 +256,7 @@ __global__ void k_vadv(Bufs b, SphConst c) {
     // XSPH: v_adv += xsph * sum_j (v_j - v_i) * ((m / rho_j) * W_ij)
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     for_neighbors<kBatch>(
-        b.nbr_ff, n, i, nf, [&](int j, long) { return F4x2{pos[j], vel[j]}; },
+        b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{pos[j], vel[j]}; },
         [&](const F4x2 &v) {
           const float4 o = v.a, vj = v.b;
           const float coef = c.mass / o.w * W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z));
@@ -294,7 +276,7 @@ __global__ void k_vadv(Bufs b, SphConst c) {
 template <bool kLds>
 __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
   const Tabs T = stage_tabs<kLds>(b, c);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = particle_index();
   if (i >= b.n) return;
   const int n = b.n;
   const float dt = (float)b.st->dt;
@@ -311,22 +293,20 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
   const float4 *__restrict__ bvel = b.bvel;
   float tr1 = 0.0f, tr2 = 0.0f;
   float dx1 = 0.0f, dy1 = 0.0f, dz1 = 0.0f, dx2 = 0.0f, dy2 = 0.0f, dz2 = 0.0f;
-  const float *__restrict__ sgf = b.sg_ff;
-  const float *__restrict__ sgs = b.sg_fs;
   for_neighbors<kBatch>(
-      b.nbr_ff, n, i, nf, [&](int j, long q) { return G2{pos[j], vadv[j], sgf[q]}; },
-      [&](const G2 &v) {
-        const float3 gw = grad_from(v.s, sub3(v.a, p));
+      b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{pos[j], vadv[j]}; },
+      [&](const F4x2 &v) {
+        const float3 gw = GW_OF(T, sub3(v.a, p));
         tr1 += dot3(make_float3(va.x - v.b.x, va.y - v.b.y, va.z - v.b.z), gw);
         dx1 = dx1 - gw.x;
         dy1 = dy1 - gw.y;
         dz1 = dz1 - gw.z;
       });
   for_neighbors<kBatch>(
-      b.nbr_fs, n, i, ns, [&](int j, long q) { return G2{bpos[j], bvel[j], sgs[q]}; },
-      [&](const G2 &v) {
+      b.nbr_fs, b.n, i, ns, [&](int j, long) { return F4x2{bpos[j], bvel[j]}; },
+      [&](const F4x2 &v) {
         const float4 o = v.a, vb = v.b;
-        const float3 gw = grad_from(v.s, sub3(o, p));
+        const float3 gw = GW_OF(T, sub3(o, p));
         tr2 += dot3(make_float3(va.x - vb.x, va.y - vb.y, va.z - vb.z), gw) * o.w;
         dx2 = dx2 - gw.x * o.w;
         dy2 = dy2 - gw.y * o.w;
@@ -338,18 +318,20 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
   const float diz = (dz1 * c.mass + dz2) * dt2 / ri2;
   float ta1 = 0.0f, ta2 = 0.0f;
   for_neighbors<kBatch>(
-      b.nbr_ff, n, i, nf, [&](int j, long q) { return G1{pos[j], sgf[q]}; },
-      [&](const G1 &v) {
-        const float3 gw = grad_from(v.s, sub3(v.a, p));
+      b.nbr_ff, b.n, i, nf, [&](int j, long) { return pos[j]; },
+      [&](const float4 &va_) {
+        const F4x2 v{va_, va_};
+        const float3 gw = GW_OF(T, sub3(v.a, p));
         const float3 dd = make_float3(dix - div_by(gw.x * t3, ri2, yi),
                                       diy - div_by(gw.y * t3, ri2, yi),
                                       diz - div_by(gw.z * t3, ri2, yi));
         ta1 += dot3(dd, gw);
       });
   for_neighbors<kBatch>(
-      b.nbr_fs, n, i, ns, [&](int j, long q) { return G1{bpos[j], sgs[q]}; },
-      [&](const G1 &v) {
-        const float3 gw = grad_from(v.s, sub3(v.a, p));
+      b.nbr_fs, b.n, i, ns, [&](int j, long) { return bpos[j]; },
+      [&](const float4 &va_) {
+        const F4x2 v{va_, va_};
+        const float3 gw = GW_OF(T, sub3(v.a, p));
         ta2 += dot3(make_float3(dix, diy, diz), gw) * v.a.w;
       });
   const float a_ii = ta1 * c.mass + ta2;
@@ -379,12 +361,11 @@ __device__ __forceinline__ void j1_particle(const Bufs &b, const SphConst &c, co
   const float ri2 = p.w * p.w;
   const float yi = 1.0f / ri2;
   float t1 = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  const float *__restrict__ sgf = b.sg_ff;
-  for_neighbors<kBatch>(
-      b.nbr_ff, n, i, b.cnt_ff[i], [&](int j, long q) { return G2{pos[j], dp[j], sgf[q]}; },
-      [&](const G2 &v) {
+  for_neighbors<kBatchJ1>(
+      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return F4x2{pos[j], dp[j]}; },
+      [&](const F4x2 &v) {
         const float4 o = v.a, dj = v.b;
-        const float3 gw = grad_from(v.s, sub3(o, p));
+        const float3 gw = GW_OF(T, sub3(o, p));
         // d_ji = grad W_ij * t3 / rho_i^2, d_ij = -grad W_ij * t3 / rho_j^2 (IISPH.py:372-373)
         const float3 term = make_float3(div_by(gw.x * t3, ri2, yi) * pi - dj.x * dj.w,
                                         div_by(gw.y * t3, ri2, yi) * pi - dj.y * dj.w,
@@ -411,18 +392,17 @@ __device__ __forceinline__ void j2_particle(const Bufs &b, const SphConst &c, co
   const float4 *__restrict__ sap = b.sap;
   const float4 *__restrict__ bpos = b.bpos;
   float t1 = 0.0f, t2 = 0.0f;
-  const float *__restrict__ sgf = b.sg_ff;
-  const float *__restrict__ sgs = b.sg_fs;
-  for_neighbors<kBatch>(
-      b.nbr_ff, n, i, b.cnt_ff[i], [&](int j, long q) { return G2{pos[j], sap[j], sgf[q]}; },
-      [&](const G2 &v) {
-        const float3 gw = grad_from(v.s, sub3(v.a, p));
+  for_neighbors<kBatchJ2>(
+      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return F4x2{pos[j], sap[j]}; },
+      [&](const F4x2 &v) {
+        const float3 gw = GW_OF(T, sub3(v.a, p));
         t1 += dot3(make_float3(si.x - v.b.x, si.y - v.b.y, si.z - v.b.z), gw);
       });
-  for_neighbors<kBatch>(
-      b.nbr_fs, n, i, b.cnt_fs[i], [&](int j, long q) { return G1{bpos[j], sgs[q]}; },
-      [&](const G1 &v) {
-        const float3 gw = grad_from(v.s, sub3(v.a, p));
+  for_neighbors<kBatchJ2>(
+      b.nbr_fs, b.n, i, b.cnt_fs[i], [&](int j, long) { return bpos[j]; },
+      [&](const float4 &va_) {
+        const F4x2 v{va_, va_};
+        const float3 gw = GW_OF(T, sub3(v.a, p));
         t2 += dot3(make_float3(si.x, si.y, si.z), gw) * v.a.w;
       });
   const float4 di = b.dp[src][i];
@@ -464,10 +444,84 @@ __device__ __forceinline__ void block_residual_add(DevState *st, int slot,
   __syncthreads();
 }
 
+// Block partial of the residual sum and count (no atomics: one slot per block).
+__device__ __forceinline__ void block_residual_store(const Bufs &b, unsigned long long rsum,
+                                                     unsigned rcnt) {
+  __shared__ unsigned long long s_sum[16];
+  __shared__ unsigned s_cnt[16];
+  rsum = wave_sum_u64(rsum);
+  rcnt = wave_sum_u32(rcnt);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_sum[wid] = rsum;
+    s_cnt[wid] = rcnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long S = 0;
+    unsigned Cn = 0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
+      S += s_sum[w];
+      Cn += s_cnt[w];
+    }
+    b.res_part[blockIdx.x] = S;
+    b.cnt_part[blockIdx.x] = Cn;
+  }
+}
+
+// Sum of the block partials of iteration `slot` into DevState; with
+// `check` set, also the stop test of the device-driven loop.
+__global__ void k_residual_reduce(Bufs b, int nparts, int slot, int check, int src0, int min_it,
+                                  int max_it, int force_it, double eta) {
+  __shared__ unsigned long long s_sum[16];
+  __shared__ unsigned s_cnt[16];
+  DevState *st = b.st;
+  if (check && st->done) return;
+  const int k = check ? st->iter : slot;
+  unsigned long long S = 0;
+  unsigned Cn = 0;
+  for (int q = threadIdx.x; q < nparts; q += blockDim.x) {
+    S += b.res_part[q];
+    Cn += b.cnt_part[q];
+  }
+  S = wave_sum_u64(S);
+  Cn = wave_sum_u32(Cn);
+  if ((threadIdx.x & 63) == 0) {
+    s_sum[threadIdx.x >> 6] = S;
+    s_cnt[threadIdx.x >> 6] = Cn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    S = 0;
+    Cn = 0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
+      S += s_sum[w];
+      Cn += s_cnt[w];
+    }
+    st->res_sum[k] = S;
+    st->res_cnt[k] = Cn;
+    if (check) {
+      double e;
+      const int cont = jacobi_continue(k, S, Cn, min_it, max_it, force_it, eta, &e);
+      if (cont) {
+        st->iter = k + 1;
+      } else {
+        st->done = 1;
+        st->iterations = k + 1;
+        st->ap_evals += k + 1;
+        st->residual = e;
+        st->converged = e <= eta;
+        st->final_src = src0 ^ (k & 1);
+        st->done_tag = st->step_tag;
+      }
+    }
+  }
+}
+
 template <bool kLds>
 __global__ void k_j1(Bufs b, SphConst c, int src) {
-  const Tabs T{b.wt, b.gt};  // grad W comes from the stored pair values
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const Tabs T = stage_tabs<kLds>(b, c);
+  const int i = particle_index();
   const float dt = (float)b.st->dt;
   const float t3 = c.mass * (dt * dt);
   if (i < b.n) j1_particle(b, c, T, i, src, t3);
@@ -475,12 +529,12 @@ __global__ void k_j1(Bufs b, SphConst c, int src) {
 
 template <bool kLds>
 __global__ void k_j2(Bufs b, SphConst c, int src, int slot) {
-  const Tabs T{b.wt, b.gt};  // grad W comes from the stored pair values
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const Tabs T = stage_tabs<kLds>(b, c);
+  const int i = particle_index();
   unsigned long long rs = 0;
   unsigned rc = 0;
   if (i < b.n) j2_particle(b, c, T, i, src, rs, rc);
-  if (slot >= 0) block_residual_add(b.st, slot, rs, rc);
+  if (slot >= 0) block_residual_store(b, rs, rc);
 }
 
 // ---- device-resident Jacobi loop control ----------------------------------
@@ -491,59 +545,26 @@ __global__ void k_j2(Bufs b, SphConst c, int src, int slot) {
 template <bool kLds>
 __global__ void k_j1_dev(Bufs b, SphConst c, int src0) {
   if (__hip_atomic_load(&b.st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-  const Tabs T{b.wt, b.gt};  // grad W comes from the stored pair values
+  const Tabs T = stage_tabs<kLds>(b, c);
   const int k = b.st->iter;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = particle_index();
   const float dt = (float)b.st->dt;
   const float t3 = c.mass * (dt * dt);
   if (i < b.n) j1_particle(b, c, T, i, src0 ^ (k & 1), t3);
 }
 
-// Stop test of iteration k (IISPH.py:1200-1207) on the residual of J2.
-__device__ __forceinline__ void jacobi_check(DevState *st, int k, int src0, int min_it,
-                                             int max_it, int force_it, double eta) {
-  const unsigned long long S =
-      __hip_atomic_load(&st->res_sum[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned Cn = __hip_atomic_load(&st->res_cnt[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  double e;
-  const int cont = jacobi_continue(k, S, Cn, min_it, max_it, force_it, eta, &e);
-  if (cont) {
-    st->iter = k + 1;
-  } else {
-    st->done = 1;
-    st->iterations = k + 1;
-    st->ap_evals += k + 1;
-    st->residual = e;
-    st->converged = e <= eta;
-    st->final_src = src0 ^ (k & 1);
-    st->done_tag = st->step_tag;
-  }
-}
-
-// J2 of a device-driven iteration; the last block to finish runs the stop test.
+// J2 of a device-driven iteration; k_residual_reduce then runs the stop test
+// (IISPH.py:1200-1207).
 template <bool kLds>
-__global__ void k_j2_dev(Bufs b, SphConst c, int src0, int min_it, int max_it, int force_it,
-                         double eta) {
-  __shared__ int s_last;
+__global__ void k_j2_dev(Bufs b, SphConst c, int src0) {
   if (__hip_atomic_load(&b.st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-  const Tabs T{b.wt, b.gt};  // grad W comes from the stored pair values
+  const Tabs T = stage_tabs<kLds>(b, c);
   const int k = b.st->iter;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = particle_index();
   unsigned long long rs = 0;
   unsigned rc = 0;
   if (i < b.n) j2_particle(b, c, T, i, src0 ^ (k & 1), rs, rc);
-  block_residual_add(b.st, k, rs, rc);
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned t = __hip_atomic_fetch_add(&b.st->blocks_done, 1u, __ATOMIC_ACQ_REL,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-    s_last = t == gridDim.x - 1;
-    if (s_last) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      b.st->blocks_done = 0u;
-      jacobi_check(b.st, k, src0, min_it, max_it, force_it, eta);
-    }
-  }
+  block_residual_store(b, rs, rc);
 }
 
 // ---- kick (IISPH.py:674-688) and drift (IISPH.py:691-699) ----------------
@@ -555,7 +576,7 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
   __shared__ unsigned s_vmax[16];
   __shared__ int s_bad[16];
   if (src < 0) src = b.st->final_src;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = particle_index();
   unsigned vbits = 0;
   int bad = 0;
   if (i < b.n) {
@@ -643,7 +664,6 @@ __global__ void k_reset_state(DevState *st) {
     st->status = 0;
     st->done = 0;
     st->step_tag += 1;
-    st->blocks_done = 0u;
     st->iter = 0;
     st->final_src = 0;
     st->residual = 0.0;
@@ -701,21 +721,25 @@ __device__ __forceinline__ bool grid_barrier(DevState *st) {
 template <bool kLds>
 __global__ void k_jacobi_persistent(Bufs b, SphConst c, int src, int min_it, int max_it,
                                     int force_it, double eta) {
-  const Tabs T{b.wt, b.gt};  // grad W comes from the stored pair values
+  const Tabs T = stage_tabs<kLds>(b, c);
   const float dt = (float)b.st->dt;
   const float t3 = c.mass * (dt * dt);
-  const int stride = gridDim.x * blockDim.x;
-  const int first = blockIdx.x * blockDim.x + threadIdx.x;
+  // each block owns a contiguous range of particles (XCD-aware order)
+  const int per_block = ((b.n + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
+  const int lo = xcd_tile() * per_block;
+  const int hi = min(b.n, lo + per_block);
+  const int stride = blockDim.x;
+  const int first = lo + threadIdx.x;
   int cur = src;
   int k = 0;
   double e = 0.0;
   bool ok = true;
   for (;; ++k) {
-    for (int i = first; i < b.n; i += stride) j1_particle(b, c, T, i, cur, t3);
+    for (int i = first; i < hi; i += stride) j1_particle(b, c, T, i, cur, t3);
     if (!(ok = grid_barrier(b.st))) break;
     unsigned long long rs = 0;
     unsigned rc = 0;
-    for (int i = first; i < b.n; i += stride) j2_particle(b, c, T, i, cur, rs, rc);
+    for (int i = first; i < hi; i += stride) j2_particle(b, c, T, i, cur, rs, rc);
     block_residual_add(b.st, k, rs, rc);
     if (!(ok = grid_barrier(b.st))) break;
     const unsigned long long S =
@@ -727,7 +751,7 @@ __global__ void k_jacobi_persistent(Bufs b, SphConst c, int src, int min_it, int
   }
   // leave the final iterate in dp[0]
   if (ok && cur == 1)
-    for (int i = first; i < b.n; i += stride) b.dp[0][i] = b.dp[1][i];
+    for (int i = first; i < hi; i += stride) b.dp[0][i] = b.dp[1][i];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     b.st->iterations = k + 1;
     b.st->ap_evals += k + 1;
@@ -766,7 +790,7 @@ void launch_boundary_psi(const Bufs &b, const SphConst &c, hipStream_t s) {
 
 void launch_lists(const Bufs &b, const SphConst &c, int block, hipStream_t s) {
   if (b.n == 0) return;
-  DISPATCH_LDS(k_lists, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c);
+  k_lists<<<nblocks(b.n, block), block, 0, s>>>(b, c);
 }
 
 void launch_density(const Bufs &b, const SphConst &c, bool fuse_vadv, int block, hipStream_t s) {
@@ -792,6 +816,8 @@ void launch_j1(const Bufs &b, const SphConst &c, int src, int block, hipStream_t
 void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, hipStream_t s) {
   if (b.n == 0) return;
   DISPATCH_LDS(k_j2, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src, slot);
+  if (slot >= 0)
+    k_residual_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block), slot, 0, 0, 0, 0, 0, 0.0);
 }
 
 void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P, int src0,
@@ -799,8 +825,9 @@ void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P,
   if (b.n == 0) return;
   for (int k = 0; k < iters; ++k) {
     DISPATCH_LDS(k_j1_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0);
-    DISPATCH_LDS(k_j2_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0,
-                 P.min_iters, P.max_iters, P.force_iters, P.eta);
+    DISPATCH_LDS(k_j2_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0);
+    k_residual_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block), 0, 1, src0, P.min_iters,
+                                         P.max_iters, P.force_iters, P.eta);
   }
 }
 

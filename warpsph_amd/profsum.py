@@ -1,0 +1,99 @@
+"""Summaries of rocprofv3 result databases (rocpd SQLite).
+
+    python -m warpsph_amd.profsum kernels <results.db> [--steps N]
+    python -m warpsph_amd.profsum traffic <fetch.db> <write.db> <copy_fetch.db> <copy_write.db> [--copy-bytes B]
+
+``kernels`` prints per-kernel call counts and durations (the --stats view);
+``traffic`` prints HBM bytes per dispatch of the Jacobi kernels from
+FETCH_SIZE and WRITE_SIZE runs, scaled by what the same counters read for a
+plain device copy of known size.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import json
+import sqlite3
+
+
+def kernels(db: str, steps: int | None = None, limit: int = 25) -> str:
+    cur = sqlite3.connect(db).cursor()
+    rows = list(cur.execute(
+        "select name, total_calls, total_duration, average, percentage from top_kernels"))
+    out = [f"{'kernel':64s} {'calls':>6s} {'total_us':>11s} {'avg_us':>9s} {'%':>6s}"
+           + ("  us/step" if steps else "")]
+    for name, calls, tot, avg, pct in rows[:limit]:
+        line = f"{name[:64]:64s} {calls:6d} {tot:11.1f} {avg:9.2f} {pct:6.2f}"
+        if steps:
+            line += f"  {tot / steps:8.1f}"
+        out.append(line)
+    return "\n".join(out)
+
+
+def _counter(db: str, counter: str):
+    """{kernel name: [value per dispatch]} of one counter (summed over instances)."""
+    cur = sqlite3.connect(db).cursor()
+    per = collections.defaultdict(float)
+    names = {}
+    for d, k, c, v in cur.execute(
+            "select dispatch_id, kernel_name, counter_name, value from counters_collection"):
+        if c == counter:
+            per[d] += v
+            names[d] = k.split("(")[0]
+    res = collections.defaultdict(list)
+    for d, v in per.items():
+        res[names[d]].append(v)
+    return res
+
+
+def traffic(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_bytes):
+    f, w = _counter(fetch_db, "FETCH_SIZE"), _counter(write_db, "WRITE_SIZE")
+    cf, cw = _counter(copy_fetch_db, "FETCH_SIZE"), _counter(copy_write_db, "WRITE_SIZE")
+
+    def big(d):  # the copy kernel's dispatches of the calibration run
+        vals = [v for vs in d.values() for v in vs]
+        return max(vals) if vals else 0.0
+    kib_f, kib_w = big(cf), big(cw)
+    scale_f = copy_bytes / (kib_f * 1024) if kib_f else float("nan")
+    scale_w = copy_bytes / (kib_w * 1024) if kib_w else float("nan")
+    out = {"copy_bytes": copy_bytes, "copy_fetch_kib": kib_f, "copy_write_kib": kib_w,
+           "fetch_scale": scale_f, "write_scale": scale_w, "kernels": {}}
+    for k in sorted(set(f) | set(w)):
+        fv = [v for v in f.get(k, []) if v > 0]
+        wv = [v for v in w.get(k, []) if v > 0]
+        if not fv and not wv:
+            continue
+        # dispatches doing real work (the device-driven Jacobi loop also
+        # issues launches that return at once after the stop test)
+        mf = max(fv) if fv else 0.0
+        mw = max(wv) if wv else 0.0
+        rf = [v for v in fv if v > 0.5 * mf]
+        rw = [v for v in wv if v > 0.5 * mw]
+        out["kernels"][k] = {"dispatches": max(len(fv), len(wv)), "working_dispatches": len(rf),
+                             "fetch_bytes_mean": sum(rf) / max(1, len(rf)) * 1024 * scale_f,
+                             "write_bytes_mean": sum(rw) / max(1, len(rw)) * 1024 * scale_w}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    k = sub.add_parser("kernels")
+    k.add_argument("db")
+    k.add_argument("--steps", type=int)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch")
+    t.add_argument("write")
+    t.add_argument("copy_fetch")
+    t.add_argument("copy_write")
+    t.add_argument("--copy-bytes", type=int, default=1024 * 1024 * 1024)
+    a = ap.parse_args()
+    if a.cmd == "kernels":
+        print(kernels(a.db, a.steps))
+    else:
+        print(json.dumps(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes),
+                         indent=1))
+
+
+if __name__ == "__main__":
+    main()
