@@ -75,7 +75,7 @@ def calibrate_copy(mib: int) -> None:
 def traffic_from_profile(workload: str):
     """HBM bytes per launch of the Jacobi kernel from the committed rocprofv3
     counter summary, if one exists for this workload."""
-    f = ROOT / "profiles" / f"jacobi_persistent_{workload}_traffic.json"
+    f = ROOT / "profiles" / f"jacobi_{workload}_traffic.json"
     if not f.exists():
         return None
     try:

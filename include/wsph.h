@@ -124,6 +124,20 @@ typedef struct wsph_step_stats {
   double v_max;     /* max |v| after the kick (IISPH.py:688) */
 } wsph_step_stats;
 
+/* Per-step record of the statistics history kept on the device. */
+typedef struct wsph_step_record {
+  int step;             /* step index since wsph_create */
+  int iterations;       /* Jacobi iterations */
+  int converged;        /* residual <= eta */
+  int pad;
+  double sim_time;      /* simulated time after the step */
+  double residual;      /* mean density error at the end of the Jacobi loop */
+  double kinetic_energy;/* 0.5 m sum |v|^2 after the step */
+  double mean_height;   /* mean fluid y */
+  double surface_height;/* highest fluid particle y (free-surface height) */
+  double v_max;         /* max |v| */
+} wsph_step_record;
+
 typedef struct wsph_solver wsph_solver;
 
 const char *wsph_last_error(void);
@@ -199,6 +213,11 @@ int wsph_set_timing(wsph_solver *s, int enable);
  * out[4] OR of the device status bits, out[5] most blocks seen at a
  * persistent-kernel barrier wait that timed out, out[6] persistent grid. */
 int wsph_get_counters(wsph_solver *s, long long *out, int nout);
+
+/* The last min(max_records, steps, 8192) step records, oldest first. The
+ * reference prints none of these (it has no statistics export); they are
+ * kept on the device so that long runs need no per-step synchronisation. */
+int wsph_get_history(wsph_solver *s, wsph_step_record *out, int max_records, int *count);
 
 /* Block until the solver's stream is idle. */
 int wsph_sync(wsph_solver *s);

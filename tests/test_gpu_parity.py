@@ -196,3 +196,35 @@ def test_drivers_agree_at_full_size():
         assert got[0] == ref[0], (mode, got[0], ref[0])
         for a, b in zip(got[1:], ref[1:]):
             assert_same(f"mode {mode}", a, b)
+
+
+@pytest.mark.parametrize("warm", [config.WARM_ZERO, config.WARM_PREVIOUS])
+def test_dambreak_200_step_statistics(orc, warm):
+    """200 steps of the 8,000-particle dam break: per-step Jacobi iteration
+    counts and residuals equal the oracle's, and the statistics (mean density
+    error, kinetic energy, free-surface and mean height) agree to well under
+    the 1% bar (the trajectories are bit-identical; only the float64
+    summation order of the statistics differs)."""
+    cfg = config.dambreak_8k().replace(warm_start=warm)
+    _announce(f"200-step statistics warm={warm}")
+    s = gpu_solver(cfg)
+    sc = scene.build(cfg)
+    O = orc.OracleSolver(cfg, sc)
+    ref = []
+    for k in range(200):
+        s.step()
+        info = O.step()
+        v = O.v.astype(np.float64)
+        ref.append(dict(iterations=info["iterations"], residual=info["residual"],
+                        kinetic_energy=0.5 * float(np.float32(cfg.mass)) * float((v * v).sum()),
+                        surface_height=float(O.x[:, 1].max()),
+                        mean_height=float(O.x[:, 1].astype(np.float64).mean())))
+    hist = s.history()[-200:]
+    assert len(hist) == 200
+    for k, (g, r) in enumerate(zip(hist, ref)):
+        assert g["iterations"] == r["iterations"], k
+        assert g["residual"] == r["residual"], k
+        assert g["surface_height"] == r["surface_height"], k
+        for key in ("kinetic_energy", "mean_height"):
+            assert abs(g[key] - r[key]) <= 1e-9 * abs(r[key]), (k, key, g[key], r[key])
+    assert_same("x", s.x.cpu().numpy(), O.x)

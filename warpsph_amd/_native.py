@@ -84,6 +84,14 @@ def make_params(cfg: SolverConfig, grid_dims, bgrid_dims) -> WsphParams:
     return p
 
 
+class WsphStepRecord(C.Structure):
+    _fields_ = [
+        ("step", C.c_int), ("iterations", C.c_int), ("converged", C.c_int), ("pad", C.c_int),
+        ("sim_time", C.c_double), ("residual", C.c_double), ("kinetic_energy", C.c_double),
+        ("mean_height", C.c_double), ("surface_height", C.c_double), ("v_max", C.c_double),
+    ]
+
+
 _P = C.c_void_p
 _I = C.c_int
 
@@ -110,6 +118,7 @@ SIGNATURES = [
     ("wsph_last_timings", _I, [_P, C.POINTER(C.c_double), _I]),
     ("wsph_set_timing", _I, [_P, _I]),
     ("wsph_get_counters", _I, [_P, C.POINTER(C.c_longlong), _I]),
+    ("wsph_get_history", _I, [_P, C.POINTER(WsphStepRecord), _I, C.POINTER(C.c_int)]),
     ("wsph_sync", _I, [_P]),
 ]
 

@@ -140,7 +140,9 @@ struct wsph_solver {
   int cur = 0;
   float4 *vadv = nullptr, *dp[2] = {nullptr, nullptr}, *sap = nullptr, *aux = nullptr;
   float *ap = nullptr, *pstart = nullptr, *p_ext = nullptr, *pfinal = nullptr;
-  unsigned *vmax_part = nullptr;
+  unsigned *vmax_part = nullptr, *ymax_part = nullptr;
+  double *ke_part = nullptr, *y_part = nullptr;
+  StepRecord *history = nullptr;
   unsigned long long *res_part = nullptr;
   unsigned *cnt_part = nullptr;
   bool have_p_ext = false;
@@ -228,6 +230,10 @@ static Bufs make_bufs(wsph_solver *s) {
   b.pstart = s->pstart;
   b.pfinal = s->pfinal;
   b.vmax_part = s->vmax_part;
+  b.ymax_part = s->ymax_part;
+  b.ke_part = s->ke_part;
+  b.y_part = s->y_part;
+  b.history = s->history;
   b.res_part = s->res_part;
   b.cnt_part = s->cnt_part;
   b.fstart = s->fg.start;
@@ -311,6 +317,7 @@ int wsph_create(const wsph_params *params, wsph_solver **out) {
   CREATE_TRY(dalloc(&s->st, 1));
   CREATE_TRY(hipHostMalloc((void **)&s->st_host, sizeof(DevState), hipHostMallocDefault));
   CREATE_TRY(dalloc(&s->totals, 8));
+  CREATE_TRY(dalloc(&s->history, kHistory));
   CREATE_TRY(hipMemsetAsync(s->totals, 0, sizeof(long long) * 8, s->stream));
   CREATE_TRY(hipMemsetAsync(s->st, 0, sizeof(DevState), s->stream));
   for (auto &e : s->ev) CREATE_TRY(hipEventCreate(&e));
@@ -347,6 +354,9 @@ static void free_fluid(wsph_solver *s) {
   dfree(s->p_ext);
   dfree(s->pfinal);
   dfree(s->vmax_part);
+  dfree(s->ymax_part);
+  dfree(s->ke_part);
+  dfree(s->y_part);
   dfree(s->res_part);
   dfree(s->cnt_part);
   dfree(s->slot);
@@ -371,6 +381,7 @@ int wsph_destroy(wsph_solver *s) {
   dfree(s->gt);
   dfree(s->st);
   dfree(s->totals);
+  dfree(s->history);
   if (s->st_host) (void)hipHostFree(s->st_host);
   for (GridBufs *g : {&s->fg, &s->bg}) {
     dfree(g->count);
@@ -496,6 +507,9 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
     HIP_TRY(dalloc(&s->pstart, n));
     HIP_TRY(dalloc(&s->pfinal, n));
     HIP_TRY(dalloc(&s->vmax_part, (size_t)(n + 63) / 64 + 1));
+    HIP_TRY(dalloc(&s->ymax_part, (size_t)(n + 63) / 64 + 1));
+    HIP_TRY(dalloc(&s->ke_part, (size_t)(n + 63) / 64 + 1));
+    HIP_TRY(dalloc(&s->y_part, (size_t)(n + 63) / 64 + 1));
     HIP_TRY(dalloc(&s->res_part, (size_t)(n + 63) / 64 + 1));
     HIP_TRY(dalloc(&s->cnt_part, (size_t)(n + 63) / 64 + 1));
     HIP_TRY(dalloc(&s->p_ext, n));
@@ -794,6 +808,31 @@ int wsph_get_counters(wsph_solver *s, long long *out, int nout) {
   HIP_TRY(hipMemcpyAsync(h, s->totals, sizeof(h), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   for (int k = 0; k < nout && k < 8; ++k) out[k] = h[k];
+  return WSPH_OK;
+}
+
+int wsph_get_history(wsph_solver *s, wsph_step_record *out, int max_records, int *count) {
+  if (!s || !out || max_records < 0) return fail(WSPH_E_ARG, "bad arguments");
+  HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const int total = s->st_host->step_count;
+  const int avail = total < kHistory ? total : kHistory;
+  const int m = avail < max_records ? avail : max_records;
+  std::vector<StepRecord> h(kHistory);
+  HIP_TRY(hipMemcpy(h.data(), s->history, sizeof(StepRecord) * kHistory, hipMemcpyDeviceToHost));
+  for (int k = 0; k < m; ++k) {
+    const StepRecord &r = h[(total - m + k) % kHistory];
+    out[k].step = total - m + k;
+    out[k].iterations = r.iterations;
+    out[k].converged = r.converged;
+    out[k].sim_time = r.sim_time;
+    out[k].residual = r.residual;
+    out[k].kinetic_energy = r.kinetic;
+    out[k].mean_height = r.mean_y;
+    out[k].surface_height = r.max_y;
+    out[k].v_max = r.v_max;
+  }
+  if (count) *count = m;
   return WSPH_OK;
 }
 

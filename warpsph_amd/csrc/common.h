@@ -14,6 +14,19 @@
 namespace wsph {
 
 constexpr int kMaxIters = 128;      // upper bound of max_iters / force_iters
+constexpr int kHistory = 8192;      // per-step statistics kept on the device (ring)
+
+// Statistics of one step (written by the device, read back on request).
+struct StepRecord {
+  double sim_time;   // after the step
+  double residual;   // mean density error at the end of the Jacobi loop
+  double kinetic;    // 0.5 m sum |v|^2 after the step
+  double mean_y;     // mean height of the fluid
+  float max_y;       // free-surface height: highest fluid particle
+  float v_max;       // max |v|
+  int iterations;    // Jacobi iterations
+  int converged;     // residual <= eta
+};
 constexpr int kMaxLdsTable = 16384; // floats per table kept in LDS (64 KiB)
 
 // status bits accumulated on the device during a step
@@ -56,6 +69,7 @@ struct DevState {
   int done_tag;               // step_tag of the step whose Jacobi loop finished last
   unsigned bar_count, bar_gen;  // grid barrier
   unsigned bar_seen;            // arrivals seen by a barrier wait that timed out
+  int step_count;             // steps recorded in the history ring
   unsigned long long res_sum[kMaxIters];
   unsigned res_cnt[kMaxIters];
 };

@@ -44,6 +44,10 @@ struct Bufs {
   float *pprev;     // pressure carried to the next step's start value
   float *pfinal;    // final pressure of the last step
   unsigned *vmax_part;  // per-block max |v| of k_integrate
+  double *ke_part;      // per-block kinetic energy of k_integrate
+  double *y_part;       // per-block sum of heights
+  unsigned *ymax_part;  // per-block max height (ordered bits)
+  StepRecord *history;  // kHistory entries
   unsigned long long *res_part;  // per-block residual sums of J2
   unsigned *cnt_part;            // per-block residual counts of J2
   const float *pstart;  // caller-supplied start pressure, sorted order (or null)

@@ -572,9 +572,22 @@ __global__ void k_j2_dev(Bufs b, SphConst c, int src0) {
 // next step's start value (pprev): zero where a_ii fell inside the guard band
 // of update_p (|a_ii| < INV_SMALL, IISPH.py:535), because the pressure of such
 // a weakly coupled particle grows without bound and means nothing.
+// float -> unsigned with the same order (for max via integer max)
+__device__ __forceinline__ unsigned ordered_bits(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered_bits(unsigned u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
 __global__ void k_integrate(Bufs b, SphConst c, int src) {
   __shared__ unsigned s_vmax[16];
   __shared__ int s_bad[16];
+  __shared__ double s_ke[16], s_y[16];
+  __shared__ unsigned s_ymax[16];
+  double ke = 0.0, ysum = 0.0;
+  unsigned ymax = 0u;
   if (src < 0) src = b.st->final_src;
   const int i = particle_index();
   unsigned vbits = 0;
@@ -600,39 +613,97 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
     b.pprev[i] = (a > -c.inv_small && a < c.inv_small) ? 0.0f : di.w;
     bad = !(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(l));
     if (bad) vbits = 0;
+    ke = 0.5 * (double)c.mass * ((double)vx * vx + (double)vy * vy + (double)vz * vz);
+    ysum = (double)y;
+    ymax = ordered_bits(y);
   }
   vbits = wave_max_u32(vbits);
   bad = wave_max_i32(bad);
+  ymax = wave_max_u32(ymax);
+  for (int o = 32; o > 0; o >>= 1) {
+    ke += __shfl_xor(ke, o, 64);
+    ysum += __shfl_xor(ysum, o, 64);
+  }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) {
     s_vmax[wid] = vbits;
     s_bad[wid] = bad;
+    s_ke[wid] = ke;
+    s_y[wid] = ysum;
+    s_ymax[wid] = ymax;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned m = 0;
+    unsigned m = 0, my = 0;
     int bb = 0;
+    double k2 = 0.0, y2 = 0.0;
     for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
       m = max(m, s_vmax[w]);
+      my = max(my, s_ymax[w]);
       bb |= s_bad[w];
+      k2 += s_ke[w];
+      y2 += s_y[w];
     }
     b.vmax_part[blockIdx.x] = m;
+    b.ymax_part[blockIdx.x] = my;
+    b.ke_part[blockIdx.x] = k2;
+    b.y_part[blockIdx.x] = y2;
     if (bb) atomicOr(&b.st->status, (int)ST_NAN);
   }
 }
 
-// max |v| over the per-block partial maxima of k_integrate
+// max |v| and the step statistics from the per-block partials of
+// k_integrate (summed in block order: deterministic), appended to the
+// history ring.
 __global__ void k_vmax_reduce(Bufs b, int nparts) {
-  unsigned m = 0;
-  for (int k = threadIdx.x; k < nparts; k += blockDim.x) m = max(m, b.vmax_part[k]);
+  __shared__ unsigned s[16], sy[16];
+  __shared__ double sk[16], ss[16];
+  unsigned m = 0, my = 0;
+  double ke = 0.0, ys = 0.0;
+  // fixed assignment of partials to threads and a fixed combining tree:
+  // the same sum on every run
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+    m = max(m, b.vmax_part[k]);
+    my = max(my, b.ymax_part[k]);
+    ke += b.ke_part[k];
+    ys += b.y_part[k];
+  }
   m = wave_max_u32(m);
-  __shared__ unsigned s[16];
-  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+  my = wave_max_u32(my);
+  for (int o = 32; o > 0; o >>= 1) {
+    ke += __shfl_xor(ke, o, 64);
+    ys += __shfl_xor(ys, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s[threadIdx.x >> 6] = m;
+    sy[threadIdx.x >> 6] = my;
+    sk[threadIdx.x >> 6] = ke;
+    ss[threadIdx.x >> 6] = ys;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w) m = max(m, s[w]);
-    m = max(m, s[0]);
-    b.st->vmax_bits = m;
+    m = 0;
+    my = 0;
+    ke = 0.0;
+    ys = 0.0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
+      m = max(m, s[w]);
+      my = max(my, sy[w]);
+      ke += sk[w];
+      ys += ss[w];
+    }
+    DevState *st = b.st;
+    st->vmax_bits = m;
+    StepRecord &r = b.history[st->step_count % kHistory];
+    r.sim_time = st->sim_time + st->dt;
+    r.residual = st->residual;
+    r.kinetic = ke;
+    r.mean_y = b.n > 0 ? ys / b.n : 0.0;
+    r.max_y = b.n > 0 ? from_ordered_bits(my) : 0.0f;
+    r.v_max = __uint_as_float(m);
+    r.iterations = st->iterations;
+    r.converged = st->converged;
+    st->step_count += 1;
   }
 }
 

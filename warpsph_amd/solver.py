@@ -154,6 +154,16 @@ class IISPH:
         return dict(steps=out[0], iterations=out[1], nonconverged=out[2], ap_evals=out[3],
                     status=out[4], barrier_seen=out[5], persistent_grid=out[6])
 
+    def history(self, max_records: int = 8192) -> list:
+        """Per-step statistics kept on the device: iterations, residual (mean
+        density error), kinetic energy, mean and free-surface height, v_max."""
+        recs = (N.WsphStepRecord * max_records)()
+        cnt = C.c_int()
+        N.check("wsph_get_history", self._lib.wsph_get_history(self._h, recs, max_records,
+                                                               C.byref(cnt)))
+        keys = [k for k, _ in N.WsphStepRecord._fields_ if k != "pad"]
+        return [{k: getattr(recs[i], k) for k in keys} for i in range(cnt.value)]
+
     def set_pressure(self, p) -> None:
         """Caller-supplied start pressure for the next step (original order)."""
         torch = _torch()
