@@ -219,6 +219,25 @@ int wsph_get_counters(wsph_solver *s, long long *out, int nout);
  * kept on the device so that long runs need no per-step synchronisation. */
 int wsph_get_history(wsph_solver *s, wsph_step_record *out, int max_records, int *count);
 
+/* ---- learned pressure start value (no reference counterpart) -----------
+ * Per-particle network 8 -> 64 -> 64 -> 1 with SiLU, evaluated on the
+ * matrix cores with BF16 operands and FP32 accumulation. Parameters are a
+ * float32 vector of wsph_mlp_nparams() = 4801 values: W1[64][8], b1[64],
+ * W2[64][64], b2[64], W3[64], b3. */
+typedef struct wsph_mlp wsph_mlp;
+int wsph_mlp_create(int max_batch, wsph_mlp **out);
+int wsph_mlp_destroy(wsph_mlp *m);
+int wsph_mlp_nparams(void);
+int wsph_mlp_set_params(wsph_mlp *m, const float *params);   /* host; resets Adam */
+int wsph_mlp_get_params(wsph_mlp *m, float *params);         /* host */
+/* y[i] = net(x[i][0..7]) for n rows (device pointers) */
+int wsph_mlp_forward(wsph_mlp *m, const float *x, int n, float *y);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on the loss 0.5 mean (net(x) - t)^2
+ * over n rows (device x[n][8], t[n]); lr <= 0 only evaluates. loss and grad
+ * (host, optional) receive the loss and the gradient before the update. */
+int wsph_mlp_train_step(wsph_mlp *m, const float *x, const float *t, int n, float lr,
+                        float *loss, float *grad);
+
 /* Block until the solver's stream is idle. */
 int wsph_sync(wsph_solver *s);
 

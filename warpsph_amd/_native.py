@@ -119,6 +119,13 @@ SIGNATURES = [
     ("wsph_set_timing", _I, [_P, _I]),
     ("wsph_get_counters", _I, [_P, C.POINTER(C.c_longlong), _I]),
     ("wsph_get_history", _I, [_P, C.POINTER(WsphStepRecord), _I, C.POINTER(C.c_int)]),
+    ("wsph_mlp_create", _I, [_I, C.POINTER(_P)]),
+    ("wsph_mlp_destroy", _I, [_P]),
+    ("wsph_mlp_nparams", _I, []),
+    ("wsph_mlp_set_params", _I, [_P, _P]),
+    ("wsph_mlp_get_params", _I, [_P, _P]),
+    ("wsph_mlp_forward", _I, [_P, _P, _I, _P]),
+    ("wsph_mlp_train_step", _I, [_P, _P, _P, _I, C.c_float, C.POINTER(C.c_float), _P]),
     ("wsph_sync", _I, [_P]),
 ]
 

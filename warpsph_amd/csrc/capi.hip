@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "mlp.h"
 
 using namespace wsph;
 
@@ -833,6 +834,93 @@ int wsph_get_history(wsph_solver *s, wsph_step_record *out, int max_records, int
     out[k].v_max = r.v_max;
   }
   if (count) *count = m;
+  return WSPH_OK;
+}
+
+// ---- pressure predictor network ---------------------------------------------
+struct wsph_mlp {
+  MlpDev dev;
+  float *adam_m = nullptr, *adam_v = nullptr, *loss = nullptr, *grad = nullptr;
+  hipStream_t stream = nullptr;
+  int step = 0;
+};
+
+int wsph_mlp_create(int max_batch, wsph_mlp **out) {
+  if (!out || max_batch < 1) return fail(WSPH_E_ARG, "bad arguments");
+  wsph_mlp *m = new wsph_mlp();
+  *out = nullptr;
+  hipError_t e = mlp_alloc(m->dev, max_batch);
+  if (e == hipSuccess) e = hipMalloc((void **)&m->adam_m, sizeof(float) * WSPH_MLP_NPARAM);
+  if (e == hipSuccess) e = hipMalloc((void **)&m->adam_v, sizeof(float) * WSPH_MLP_NPARAM);
+  if (e == hipSuccess) e = hipMalloc((void **)&m->loss, sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void **)&m->grad, sizeof(float) * WSPH_MLP_NPARAM);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMemsetAsync(m->adam_m, 0, sizeof(float) * WSPH_MLP_NPARAM, m->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(m->adam_v, 0, sizeof(float) * WSPH_MLP_NPARAM, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e != hipSuccess) {
+    wsph_mlp_destroy(m);
+    return fail(WSPH_E_HIP, "predictor allocation: %s", hipGetErrorString(e));
+  }
+  *out = m;
+  return WSPH_OK;
+}
+
+int wsph_mlp_destroy(wsph_mlp *m) {
+  if (!m) return WSPH_OK;
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  mlp_free(m->dev);
+  for (float *p : {m->adam_m, m->adam_v, m->loss, m->grad})
+    if (p) (void)hipFree(p);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return WSPH_OK;
+}
+
+int wsph_mlp_nparams(void) { return WSPH_MLP_NPARAM; }
+
+int wsph_mlp_set_params(wsph_mlp *m, const float *params) {
+  if (!m || !params) return fail(WSPH_E_ARG, "null argument");
+  HIP_TRY(hipMemcpyAsync(m->dev.params, params, sizeof(float) * WSPH_MLP_NPARAM,
+                         hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(hipMemsetAsync(m->adam_m, 0, sizeof(float) * WSPH_MLP_NPARAM, m->stream));
+  HIP_TRY(hipMemsetAsync(m->adam_v, 0, sizeof(float) * WSPH_MLP_NPARAM, m->stream));
+  mlp_sync_weights(m->dev, m->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  m->step = 0;
+  return WSPH_OK;
+}
+
+int wsph_mlp_get_params(wsph_mlp *m, float *params) {
+  if (!m || !params) return fail(WSPH_E_ARG, "null argument");
+  HIP_TRY(hipMemcpyAsync(params, m->dev.params, sizeof(float) * WSPH_MLP_NPARAM,
+                         hipMemcpyDeviceToHost, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return WSPH_OK;
+}
+
+int wsph_mlp_forward(wsph_mlp *m, const float *x, int n, float *y) {
+  if (!m || n < 0 || (n > 0 && (!x || !y))) return fail(WSPH_E_ARG, "bad arguments");
+  mlp_forward(m->dev, x, n, y, m->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return WSPH_OK;
+}
+
+int wsph_mlp_train_step(wsph_mlp *m, const float *x, const float *t, int n, float lr,
+                        float *loss, float *grad) {
+  if (!m || n < 1 || !x || !t) return fail(WSPH_E_ARG, "bad arguments");
+  if (n > m->dev.max_batch) return fail(WSPH_E_ARG, "batch %d above the maximum %d", n, m->dev.max_batch);
+  if (lr > 0.0f) m->step++;
+  mlp_train_step(m->dev, m->adam_m, m->adam_v, m->step > 0 ? m->step : 1, lr, x, t, n, m->loss,
+                 m->grad, m->stream);
+  HIP_TRY(hipGetLastError());
+  if (loss) HIP_TRY(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  if (grad)
+    HIP_TRY(hipMemcpyAsync(grad, m->grad, sizeof(float) * WSPH_MLP_NPARAM, hipMemcpyDeviceToHost,
+                           m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
   return WSPH_OK;
 }
 
