@@ -45,10 +45,20 @@ enum {
 /* ---- enumerations ------------------------------------------------------ */
 enum { WSPH_LAYOUT_LINEAR = 0, WSPH_LAYOUT_MORTON = 1 };
 /* pressure start value each step: 0 as IISPH.py:166-174, the previous
- * step's pressure, or a caller-supplied field (learned predictor). The
- * previous step's pressure is carried as zero for particles whose a_ii fell
- * inside update_p's guard band |a_ii| < INV_SMALL (IISPH.py:535). */
-enum { WSPH_WARM_ZERO = 0, WSPH_WARM_PREVIOUS = 1, WSPH_WARM_EXTERNAL = 2 };
+ * step's pressure, a caller-supplied field, or the attached pressure
+ * predictor network (wsph_set_predictor). The previous step's pressure is
+ * carried as zero for particles whose a_ii fell inside update_p's guard band
+ * |a_ii| < INV_SMALL (IISPH.py:535).
+ * WSPH_WARM_LEARNED: at step start a feature pass (8 features per particle,
+ * see wsph_get_pairs) feeds the network, evaluated inline on the matrix
+ * cores; the start value is max(0, p_carry + P_ref * net), the carried value
+ * where |a_ii / a_ref| < 0.05 (weakly coupled particles), 0 inside the guard
+ * band. The previous step's pressure judges the network: after each step
+ * both start values are compared with the converged pressure, and when the
+ * network's error was the larger one the next step starts from the carried
+ * pressure instead (the network is still evaluated and judged). */
+enum { WSPH_WARM_ZERO = 0, WSPH_WARM_PREVIOUS = 1, WSPH_WARM_EXTERNAL = 2,
+       WSPH_WARM_LEARNED = 3 };
 /* dt: CFL rule of IISPH.py:1368-1370, or fixed */
 enum { WSPH_DT_CFL = 0, WSPH_DT_FIXED = 1 };
 /* Jacobi driver: one launch per kernel with a host convergence test per
@@ -129,13 +139,20 @@ typedef struct wsph_step_record {
   int step;             /* step index since wsph_create */
   int iterations;       /* Jacobi iterations */
   int converged;        /* residual <= eta */
-  int pad;
+  int start_source;     /* start value used: WSPH_WARM_* (LEARNED only when the
+                           network's value was used) */
   double sim_time;      /* simulated time after the step */
   double residual;      /* mean density error at the end of the Jacobi loop */
   double kinetic_energy;/* 0.5 m sum |v|^2 after the step */
   double mean_height;   /* mean fluid y */
   double surface_height;/* highest fluid particle y (free-surface height) */
   double v_max;         /* max |v| */
+  /* sqrt(mean_i (w_i (start value_i - converged p_i))^2) [Pa] with
+   * w_i = min(|a_ii / a_ref|, 1), a particle's weight in the density
+   * residual, for the network's prediction and for the carried previous
+   * pressure; computed in WSPH_WARM_LEARNED mode, else 0 */
+  double start_err_learned;
+  double start_err_carry;
 } wsph_step_record;
 
 typedef struct wsph_solver wsph_solver;
@@ -237,6 +254,25 @@ int wsph_mlp_forward(wsph_mlp *m, const float *x, int n, float *y);
  * (host, optional) receive the loss and the gradient before the update. */
 int wsph_mlp_train_step(wsph_mlp *m, const float *x, const float *t, int n, float lr,
                         float *loss, float *grad);
+
+/* Attach a predictor network (kept by reference, not owned; NULL detaches)
+ * for WSPH_WARM_LEARNED. No reference counterpart. */
+int wsph_set_predictor(wsph_solver *s, wsph_mlp *m);
+/* Keep each step's feature rows and regression targets (pair logging) for
+ * wsph_get_pairs (on != 0), at 36 bytes per particle of device memory. */
+int wsph_set_pair_logging(wsph_solver *s, int on);
+/* Training pairs of the last step, in original particle order:
+ * features[n][8] (device) and targets[n] (device) with
+ * target = (converged p - carried p) / P_ref, the quantity the network
+ * predicts. P_ref = rho0 * max(|g|, 1) * 0.1. Feature columns: 0 density
+ * error (rho_adv / rho0 - 1) * 100; 1 a_ii / a_ref, with a_ref the a_ii of a
+ * particle inside a resting lattice of the solver's spacing at the current
+ * dt; 2 fluid neighbour count / 32; 3 carried pressure / P_ref (columns
+ * 0-3 and the target clamped to +-64); 4-7 the kernel-weighted means of
+ * columns 0-3 over the particle and its fluid neighbours,
+ * sum_j W_ij f_j / sum_j W_ij. Requires pair logging or
+ * WSPH_WARM_LEARNED; fails before the first step. */
+int wsph_get_pairs(wsph_solver *s, float *features, float *targets);
 
 /* Block until the solver's stream is idle. */
 int wsph_sync(wsph_solver *s);

@@ -86,9 +86,11 @@ def make_params(cfg: SolverConfig, grid_dims, bgrid_dims) -> WsphParams:
 
 class WsphStepRecord(C.Structure):
     _fields_ = [
-        ("step", C.c_int), ("iterations", C.c_int), ("converged", C.c_int), ("pad", C.c_int),
+        ("step", C.c_int), ("iterations", C.c_int), ("converged", C.c_int),
+        ("start_source", C.c_int),
         ("sim_time", C.c_double), ("residual", C.c_double), ("kinetic_energy", C.c_double),
         ("mean_height", C.c_double), ("surface_height", C.c_double), ("v_max", C.c_double),
+        ("start_err_learned", C.c_double), ("start_err_carry", C.c_double),
     ]
 
 
@@ -126,6 +128,9 @@ SIGNATURES = [
     ("wsph_mlp_get_params", _I, [_P, _P]),
     ("wsph_mlp_forward", _I, [_P, _P, _I, _P]),
     ("wsph_mlp_train_step", _I, [_P, _P, _P, _I, C.c_float, C.POINTER(C.c_float), _P]),
+    ("wsph_set_predictor", _I, [_P, _P]),
+    ("wsph_set_pair_logging", _I, [_P, _I]),
+    ("wsph_get_pairs", _I, [_P, _P, _P]),
     ("wsph_sync", _I, [_P]),
 ]
 

@@ -19,6 +19,7 @@ DT_FIXED = 1
 WARM_ZERO = 0
 WARM_PREVIOUS = 1
 WARM_EXTERNAL = 2
+WARM_LEARNED = 3
 JACOBI_HOST = 0
 JACOBI_PERSISTENT = 1
 JACOBI_DEVICE = 2
@@ -152,8 +153,9 @@ def batch_256() -> SolverConfig:
 
 
 def learned_1m() -> SolverConfig:
-    """dambreak_1m with the learned pressure start value; 500 steps."""
-    return dambreak_1m().replace(warm_start=WARM_EXTERNAL)
+    """dambreak_1m with the learned pressure start value (predictor network,
+    judged against the previous step's pressure); 500 steps."""
+    return dambreak_1m().replace(warm_start=WARM_LEARNED)
 
 
 PRESETS = {

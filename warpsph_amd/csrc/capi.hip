@@ -12,6 +12,7 @@
 
 #include "kernels.h"
 #include "mlp.h"
+#include "mlp_dev.h"
 
 using namespace wsph;
 
@@ -124,6 +125,14 @@ struct GridBufs {
   int dims[3] = {0, 0, 0};
 };
 
+// ---- pressure predictor network ---------------------------------------------
+struct wsph_mlp {
+  MlpDev dev;
+  float *adam_m = nullptr, *adam_v = nullptr, *loss = nullptr, *grad = nullptr;
+  hipStream_t stream = nullptr;
+  int step = 0;
+};
+
 struct wsph_solver {
   wsph_params P;
   SphConst C;
@@ -167,6 +176,12 @@ struct wsph_solver {
   int *done_flags = nullptr;
   hipEvent_t chunk_ev[2] = {};
   int step_count = 0;
+  // learned start and training-pair logging
+  wsph_mlp *net = nullptr;
+  bool log_pairs = false, have_pairs = false;
+  float4 *seed = nullptr;
+  float *feat = nullptr, *target = nullptr;
+  double *err_part = nullptr;
 };
 
 // Smallest float32 d2 >= 0 with sqrtf(d2) >= h: then sqrtf(d2) < h exactly
@@ -253,7 +268,38 @@ static Bufs make_bufs(wsph_solver *s) {
   b.gt = s->gt;
   b.lds_tables = s->lds_tables;
   b.st = s->st;
+  const bool learned = s->P.warm_start == WSPH_WARM_LEARNED;
+  if (learned || s->log_pairs) b.seed = s->seed;
+  if (s->log_pairs) {
+    b.feat = s->feat;
+    b.target = s->target;
+  }
+  b.err_part = s->err_part;
+  b.judge = learned && s->net && s->err_part;
   return b;
+}
+
+// Scale of a_ii for the predictor's features: a particle inside a resting
+// lattice of spacing d has d_ii = 0 (symmetric neighbourhood) and
+// a_ii = -dt^2 m^2 / rho0^2 * S with S = sum_j |grad W(x_j)|^2 over the lattice
+// points within h (cubic spline of sph_funcs.py:56-82). Returns
+// -rho0^2 / (m^2 S), so that a_ii / a_ref = a_ii * k1 / dt^2.
+static float feature_k1(const wsph_params &P) {
+  const double h = P.h, d = P.spacing, sigma = 8.0 / (M_PI * h * h * h);
+  const int R = (int)(h / d) + 1;
+  double S = 0.0;
+  for (int a = -R; a <= R; ++a)
+    for (int b = -R; b <= R; ++b)
+      for (int c = -R; c <= R; ++c) {
+        const double r = d * sqrt((double)(a * a + b * b + c * c));
+        if (r <= 0.0 || r >= h) continue;
+        const double q = r / h;
+        const double g = q <= 0.5 ? (12.0 * q - 18.0 * q * q) * sigma / h
+                                  : 6.0 * (1.0 - q) * (1.0 - q) * sigma / h;
+        S += g * g;
+      }
+  if (S <= 0.0) return 0.0f;
+  return (float)(-(P.rho0 * P.rho0) / (P.mass * P.mass * S));
 }
 
 static int check_dims(const int *d, const char *what) {
@@ -282,7 +328,7 @@ int wsph_create(const wsph_params *params, wsph_solver **out) {
     return fail(WSPH_E_ARG, "block must be a multiple of 64 in [64, 1024]");
   if (P.cell_layout != WSPH_LAYOUT_LINEAR && P.cell_layout != WSPH_LAYOUT_MORTON)
     return fail(WSPH_E_ARG, "unknown cell layout");
-  if (P.warm_start < WSPH_WARM_ZERO || P.warm_start > WSPH_WARM_EXTERNAL)
+  if (P.warm_start < WSPH_WARM_ZERO || P.warm_start > WSPH_WARM_LEARNED)
     return fail(WSPH_E_ARG, "unknown warm start mode");
   int rc;
   if ((rc = check_dims(P.grid_dims, "grid")) || (rc = check_dims(P.bgrid_dims, "boundary grid")))
@@ -304,6 +350,10 @@ int wsph_create(const wsph_params *params, wsph_solver **out) {
   memcpy(C.fdims, P.grid_dims, sizeof(C.fdims));
   memcpy(C.bdims, P.bgrid_dims, sizeof(C.bdims));
   C.layout = P.cell_layout;
+  C.feat_k1 = feature_k1(P);
+  const double g_abs = fabs(P.gravity) > 1.0 ? fabs(P.gravity) : 1.0;
+  C.pref = (float)(P.rho0 * g_abs * 0.1);
+  C.inv_pref = 1.0f / C.pref;
   *out = nullptr;
 #define CREATE_TRY(x)                                                                        \
   do {                                                                                       \
@@ -368,7 +418,24 @@ static void free_fluid(wsph_solver *s) {
   dfree(s->cnt_ff);
   dfree(s->nbr_fs);
   dfree(s->cnt_fs);
+  dfree(s->seed);
+  dfree(s->feat);
+  dfree(s->target);
+  dfree(s->err_part);
+  s->have_pairs = false;
   s->n = 0;
+}
+
+// Buffers of the learned start / pair logging, allocated on first use.
+static int ensure_learn_bufs(wsph_solver *s) {
+  const size_t n = (size_t)s->cap_n;
+  if (!s->seed) HIP_TRY(dalloc(&s->seed, n));
+  if (!s->err_part) HIP_TRY(dalloc(&s->err_part, 2 * ((n + 63) / 64 + 1)));
+  if (s->log_pairs) {
+    if (!s->feat) HIP_TRY(dalloc(&s->feat, n * WSPH_MLP_IN));
+    if (!s->target) HIP_TRY(dalloc(&s->target, n));
+  }
+  return WSPH_OK;
 }
 
 int wsph_destroy(wsph_solver *s) {
@@ -592,11 +659,22 @@ int wsph_neighbor_search(wsph_solver *s) {
 int wsph_predict_advection(wsph_solver *s) {
   if (!s || !s->searched) return fail(WSPH_E_ARG, "run wsph_neighbor_search first");
   rec(s, 1);
+  const bool learned = s->P.warm_start == WSPH_WARM_LEARNED;
+  if (learned && !s->net) return fail(WSPH_E_ARG, "learned start needs wsph_set_predictor");
+  if (learned || s->log_pairs) {
+    int rc = ensure_learn_bufs(s);
+    if (rc) return rc;
+  }
   const Bufs b = make_bufs(s);
   const bool plain = s->P.vis_mu == 0.0 && s->P.xsph == 0.0;
   launch_density(b, s->C, plain, s->P.block, s->stream);
   if (!plain) launch_vadv(b, s->C, s->P.block, s->stream);
   launch_assemble(b, s->C, s->P.warm_start, s->P.block, s->stream);
+  if (learned || s->log_pairs) {
+    Weights W;
+    if (learned) W = weights_of(s->net->dev);
+    launch_learned_start(b, s->C, learned ? &W : nullptr, s->stream);
+  }
   HIP_TRY(hipGetLastError());
   s->advected = true;
   return WSPH_OK;
@@ -713,7 +791,9 @@ int wsph_integrate(wsph_solver *s, wsph_step_stats *stats) {
     HIP_TRY(hipMemcpyAsync(&s->st_host->dt, &s->st->dt, sizeof(double), hipMemcpyDeviceToHost,
                            s->stream));
   }
-  launch_integrate(b, s->C, s->final_dp, s->P.block, s->stream);  // -1: index kept on device
+  // final_dp -1: index kept on the device
+  launch_integrate(b, s->C, s->final_dp, s->P.warm_start, s->P.block, s->stream);
+  s->have_pairs = s->log_pairs && s->n > 0;
   launch_clock(b, s->P, s->stream);
   k_accumulate_totals<<<1, 1, 0, s->stream>>>(s->st, s->totals, s->persistent_grid);
   rec(s, 4);
@@ -832,18 +912,14 @@ int wsph_get_history(wsph_solver *s, wsph_step_record *out, int max_records, int
     out[k].mean_height = r.mean_y;
     out[k].surface_height = r.max_y;
     out[k].v_max = r.v_max;
+    out[k].start_source = r.start_source;
+    out[k].start_err_learned = r.err_net;
+    out[k].start_err_carry = r.err_carry;
   }
   if (count) *count = m;
   return WSPH_OK;
 }
 
-// ---- pressure predictor network ---------------------------------------------
-struct wsph_mlp {
-  MlpDev dev;
-  float *adam_m = nullptr, *adam_v = nullptr, *loss = nullptr, *grad = nullptr;
-  hipStream_t stream = nullptr;
-  int step = 0;
-};
 
 int wsph_mlp_create(int max_batch, wsph_mlp **out) {
   if (!out || max_batch < 1) return fail(WSPH_E_ARG, "bad arguments");
@@ -921,6 +997,30 @@ int wsph_mlp_train_step(wsph_mlp *m, const float *x, const float *t, int n, floa
     HIP_TRY(hipMemcpyAsync(grad, m->grad, sizeof(float) * WSPH_MLP_NPARAM, hipMemcpyDeviceToHost,
                            m->stream));
   HIP_TRY(hipStreamSynchronize(m->stream));
+  return WSPH_OK;
+}
+
+int wsph_set_predictor(wsph_solver *s, wsph_mlp *m) {
+  if (!s) return fail(WSPH_E_ARG, "null solver");
+  s->net = m;
+  return WSPH_OK;
+}
+
+int wsph_set_pair_logging(wsph_solver *s, int on) {
+  if (!s) return fail(WSPH_E_ARG, "null solver");
+  s->log_pairs = on != 0;
+  s->have_pairs = false;
+  return WSPH_OK;
+}
+
+int wsph_get_pairs(wsph_solver *s, float *features, float *targets) {
+  if (!s || !features || !targets) return fail(WSPH_E_ARG, "null argument");
+  if (!s->have_pairs) return fail(WSPH_E_ARG, "no logged step: enable pair logging and step first");
+  const int *orig = s->orig[s->cur];  // the order of the logged step
+  launch_scatter(s->feat, WSPH_MLP_IN, WSPH_MLP_IN, 0, orig, s->n, features, s->stream);
+  launch_scatter(s->target, 1, 1, 0, orig, s->n, targets, s->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
   return WSPH_OK;
 }
 

@@ -26,8 +26,19 @@ struct StepRecord {
   float v_max;       // max |v|
   int iterations;    // Jacobi iterations
   int converged;     // residual <= eta
+  int start_source;  // WSPH_WARM_* start value used
+  double err_net;    // RMS (learned start - converged p), learned mode
+  double err_carry;  // RMS (carried start - converged p), learned mode
 };
 constexpr int kMaxLdsTable = 16384; // floats per table kept in LDS (64 KiB)
+// Learned start: self features and targets are clamped to +-kFeatClamp; a
+// particle with |a_ii / a_ref| < kWeakCoupling keeps the carried pressure
+// (its pressure barely moves the density and its value is unreliable).
+constexpr float kFeatClamp = 64.0f;
+constexpr float kWeakCoupling = 0.05f;
+__host__ __device__ inline float clamp_feat(float v) {
+  return v > kFeatClamp ? kFeatClamp : (v < -kFeatClamp ? -kFeatClamp : v);
+}
 
 // status bits accumulated on the device during a step
 enum : int {
@@ -49,6 +60,9 @@ struct SphConst {
   float mass, rho0, omega, inv_small, g_y, vis_mu, xsph;
   int fdims[3], bdims[3]; // hash dims of the fluid and boundary grids
   int layout;             // WSPH_LAYOUT_*
+  // feature scales of the pressure predictor: a_ii / a_ref = a_ii * feat_k1 / dt^2,
+  // pressures in units of pref
+  float feat_k1, pref, inv_pref;
 };
 
 // Device-resident step state: clock, residual accumulators, counters.
@@ -70,6 +84,7 @@ struct DevState {
   unsigned bar_count, bar_gen;  // grid barrier
   unsigned bar_seen;            // arrivals seen by a barrier wait that timed out
   int step_count;             // steps recorded in the history ring
+  int net_off;                // learned start: 1 = the network lost the last comparison
   unsigned long long res_sum[kMaxIters];
   unsigned res_cnt[kMaxIters];
 };

@@ -51,6 +51,12 @@ struct Bufs {
   unsigned long long *res_part;  // per-block residual sums of J2
   unsigned *cnt_part;            // per-block residual counts of J2
   const float *pstart;  // caller-supplied start pressure, sorted order (or null)
+  // pressure predictor (null / 0 when unused)
+  float4 *seed;         // per-particle self features 0..3, written by k_assemble
+  float *feat;          // feature rows [n][8] (pair logging)
+  float *target;        // regression targets [n] (pair logging)
+  double *err_part;     // per-block (net, carry) squared start errors of k_integrate
+  int judge;            // learned start: compare both start values after the solve
   // fluid grid
   const unsigned *fstart, *fcount;
   // neighbour lists, column-major: list[k * n + i]
@@ -72,6 +78,10 @@ void launch_density(const Bufs &b, const SphConst &c, bool fuse_vadv, int block,
 void launch_vadv(const Bufs &b, const SphConst &c, int block, hipStream_t s);
 // start: WSPH_WARM_* ; writes dp[0] = (d_ii, p1) for a zero start, else (d_ii, p0)
 void launch_assemble(const Bufs &b, const SphConst &c, int start_mode, int block, hipStream_t s);
+// Feature pass of the learned start (learn.hip): features of every particle,
+// optionally the network's start value (net != null) into dp[0].w and aux.z.
+struct Weights;
+void launch_learned_start(const Bufs &b, const SphConst &c, const Weights *net, hipStream_t s);
 // One A p evaluation on dp[src]; writes dp[1-src] = next iterate. slot >= 0
 // accumulates the residual of iteration `slot`.
 void launch_j1(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s);
@@ -80,7 +90,8 @@ void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, h
 void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P, int src0,
                          int iters, int block, hipStream_t s);
 // src < 0: read the final buffer index from DevState
-void launch_integrate(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s);
+void launch_integrate(const Bufs &b, const SphConst &c, int src, int start_mode, int block,
+                      hipStream_t s);
 void launch_clock(const Bufs &b, const wsph_params &P, hipStream_t s);
 void launch_reset_state(const Bufs &b, hipStream_t s);
 // Persistent cooperative Jacobi loop; returns a hip error code.

@@ -1,0 +1,113 @@
+// learn.hip -- feature pass of the learned pressure start, with the
+// predictor network evaluated inline on the matrix cores.
+//
+// The reference starts every Jacobi solve from p = 0 (IISPH.py:166-174).
+// Here, after k_assemble has written each particle's self features
+// (seed = density error, a_ii / a_ref, neighbour count, carried pressure),
+// one pass per particle
+//   1. gathers the kernel-weighted mean of the four self features over the
+//      particle and its fluid neighbours (sum_j W_ij f_j / sum_j W_ij, the
+//      particle itself entering with W(0)),
+//   2. optionally writes the 8-feature row (training pairs),
+//   3. stages the rows of its 64 particles in LDS and runs the network on
+//      them as two 32-row MFMA tiles (BF16 operands, FP32 accumulation),
+//   4. writes the start value p0 = max(0, carry + P_ref * y) (the carried
+//      value where |a_ii / a_ref| < kWeakCoupling, zero inside the a_ii guard
+//      band) to aux.z (always) and to the Jacobi iterate dp[0].w
+//      unless the network lost the previous step's comparison with the
+//      carried pressure (DevState::net_off), in which case dp[0].w keeps the
+//      carried value k_assemble wrote.
+#include "kernels.h"
+#include "mlp_dev.h"
+
+namespace wsph {
+
+constexpr int kLearnWaves = 4;  // one block = 256 particles = 8 MFMA tiles
+
+struct PosSeed {
+  float4 pos, seed;
+};
+
+template <bool kInfer>
+__global__ __launch_bounds__(64 * kLearnWaves) void k_learned_start(Bufs b, SphConst c, Weights W) {
+  __shared__ float xf[kLearnWaves][64][WSPH_MLP_IN];
+  __shared__ float act[kInfer ? kLearnWaves : 1][32][65];
+  __shared__ float ys[kLearnWaves][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = particle_index();
+  const bool live = i < b.n;  // no early exit: forward_tile has block barriers
+  float f[WSPH_MLP_IN] = {};
+  float carry = 0.0f, a_ii = 0.0f;
+  if (live) {
+    const int n = b.n;
+    const float4 p = b.pos[i];
+    const float4 s = b.seed[i];
+    const float w0 = spline_w(b.wt, c.nw, c.w_scale, c.h, 0.0f);
+    float sw = w0, m0 = w0 * s.x, m1 = w0 * s.y, m2 = w0 * s.z, m3 = w0 * s.w;
+    const float4 *__restrict__ pos = b.pos;
+    const float4 *__restrict__ seed = b.seed;
+    for_neighbors<4>(
+        b.nbr_ff, n, i, b.cnt_ff[i], [&](int j, long) { return PosSeed{pos[j], seed[j]}; },
+        [&](const PosSeed &v) {
+          const float w = spline_w(b.wt, c.nw, c.w_scale, c.h,
+                                   dist3(p.x, p.y, p.z, v.pos.x, v.pos.y, v.pos.z));
+          sw += w;
+          m0 += w * v.seed.x;
+          m1 += w * v.seed.y;
+          m2 += w * v.seed.z;
+          m3 += w * v.seed.w;
+        });
+    const float inv = 1.0f / sw;
+    f[0] = s.x;
+    f[1] = s.y;
+    f[2] = s.z;
+    f[3] = s.w;
+    f[4] = m0 * inv;
+    f[5] = m1 * inv;
+    f[6] = m2 * inv;
+    f[7] = m3 * inv;
+    carry = b.pprev[i];
+    a_ii = b.aux[i].y;
+    if (b.feat) {
+      float4 *row = reinterpret_cast<float4 *>(b.feat + (long)i * WSPH_MLP_IN);
+      row[0] = make_float4(f[0], f[1], f[2], f[3]);
+      row[1] = make_float4(f[4], f[5], f[6], f[7]);
+    }
+  }
+  if constexpr (kInfer) {
+    const int net_off = b.st->net_off;
+#pragma unroll
+    for (int k = 0; k < WSPH_MLP_IN; ++k) xf[wave][lane][k] = f[k];
+    __syncthreads();
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      float y[16];
+      forward_tile(W, &xf[wave][32 * half][0], 32, act[wave], nullptr, nullptr, y);
+      if ((lane & 31) == 0)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) ys[wave][32 * half + drow(lane, k)] = y[k];
+    }
+    __syncthreads();
+    if (live) {
+      const float y = ys[wave][lane];
+      float pn = isfinite(y) ? carry + c.pref * y : carry;
+      pn = pn > 0.0f ? pn : 0.0f;
+      if (fabsf(f[1]) < kWeakCoupling) pn = carry;
+      if (a_ii > -c.inv_small && a_ii < c.inv_small) pn = 0.0f;
+      reinterpret_cast<float *>(&b.aux[i])[2] = pn;
+      if (!net_off) reinterpret_cast<float *>(&b.dp[0][i])[3] = pn;
+    }
+  }
+}
+
+void launch_learned_start(const Bufs &b, const SphConst &c, const Weights *net, hipStream_t s) {
+  if (b.n == 0) return;
+  const int block = 64 * kLearnWaves;
+  const int grid = (b.n + block - 1) / block;
+  if (net)
+    k_learned_start<true><<<grid, block, 0, s>>>(b, c, *net);
+  else
+    k_learned_start<false><<<grid, block, 0, s>>>(b, c, Weights{});
+}
+
+}  // namespace wsph

@@ -2,33 +2,13 @@
 // gfx950 matrix cores: BF16 operands, FP32 accumulation
 // (v_mfma_f32_32x32x16_bf16), forward and backward, Adam in FP32.
 //
-// One wave evaluates a tile of 32 particles. Operand layout of the 32x32x16
-// MFMA (checked on the device): A[m][k] sits in lane m + 32*(k/8), element
-// k%8; B[k][n] in lane n + 32*(k/8), element k%8; D[m][n] in lane
-// n + 32*((m/4)%2), register 4*(m/8) + m%4. Activations move between layers
-// through LDS, where they are re-read in the A (or B) layout.
+// One wave evaluates a tile of 32 rows (building blocks in mlp_dev.h).
 #include <hip/hip_runtime.h>
 #include <string.h>
 
-#include "mlp.h"
+#include "mlp_dev.h"
 
 namespace wsph {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-__device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
-  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
-}
-
-// row of accumulator register i for this lane
-__device__ __forceinline__ int drow(int lane, int i) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); }
-
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
-__device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
-  return s * (1.0f + z * (1.0f - s));
-}
 
 // Per-wave LDS scratch: 32 rows x 64 columns of float (activations) plus
 // a second buffer for pre-activations in training.
@@ -36,100 +16,6 @@ struct WaveLds {
   float a[32][65];
   float b[32][65];
 };
-
-// Load the A operand (rows 0..31, k-slice ks of 16) from a float matrix
-// m[32][ld] in LDS.
-__device__ __forceinline__ bf16x8 lds_a(const float (*m)[65], int lane, int ks) {
-  bf16x8 v;
-  const int r = lane & 31, k0 = 16 * ks + 8 * (lane >> 5);
-#pragma unroll
-  for (int j = 0; j < 8; ++j) v[j] = (__bf16)m[r][k0 + j];
-  return v;
-}
-
-// B operand of W^T for output tile nt (columns 32*nt..) and k-slice ks,
-// W stored row-major [nout][nin] in bf16 (global or LDS).
-__device__ __forceinline__ bf16x8 w_b(const __bf16 *w, int nin, int nout, int lane, int nt, int ks) {
-  bf16x8 v;
-  const int n = 32 * nt + (lane & 31), k0 = 16 * ks + 8 * (lane >> 5);
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int k = k0 + j;
-    v[j] = (n < nout && k < nin) ? w[n * nin + k] : (__bf16)0.0f;
-  }
-  return v;
-}
-
-// Forward pass of one 32-row tile. x: features (fp32, row-major, 8 per row;
-// rows past `rows` are read as zero). Leaves h2 in L.a; when z1s / z2s are
-// given, the pre-activations of layers 1 and 2 are stored there ([32][64]).
-// Returns y in y[16]: lane 0 holds rows drow(0,i), lane 32 rows drow(32,i).
-struct Weights {
-  const __bf16 *w1, *w2, *w3;  // bf16 copies [64][8], [64][64], [64]
-  const float *b1, *b2, *b3;   // fp32 biases
-};
-
-__device__ __forceinline__ void forward_tile(const Weights &W, const float *x, int rows,
-                                             WaveLds &L, float *z1s, float *z2s, float y[16]) {
-  const int lane = threadIdx.x & 63;
-  // layer 1: A = x (K = 8, padded to 16)
-  bf16x8 ax;
-  {
-    const int r = lane & 31, k0 = 8 * (lane >> 5);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = k0 + j;
-      ax[j] = (r < rows && k < WSPH_MLP_IN) ? (__bf16)x[r * WSPH_MLP_IN + k] : (__bf16)0.0f;
-    }
-  }
-#pragma unroll
-  for (int nt = 0; nt < 2; ++nt) {
-    f32x16 c;
-    const int n = 32 * nt + (lane & 31);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) c[i] = W.b1[n];
-    c = mfma(ax, w_b(W.w1, WSPH_MLP_IN, WSPH_MLP_H, lane, nt, 0), c);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int r = drow(lane, i);
-      L.a[r][n] = silu(c[i]);
-      if (z1s) z1s[r * 64 + n] = c[i];
-    }
-  }
-  __syncthreads();
-  // layer 2
-  bf16x8 ah[4];
-#pragma unroll
-  for (int ks = 0; ks < 4; ++ks) ah[ks] = lds_a(L.a, lane, ks);
-  __syncthreads();
-#pragma unroll
-  for (int nt = 0; nt < 2; ++nt) {
-    f32x16 c;
-    const int n = 32 * nt + (lane & 31);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) c[i] = W.b2[n];
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) c = mfma(ah[ks], w_b(W.w2, WSPH_MLP_H, WSPH_MLP_H, lane, nt, ks), c);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int r = drow(lane, i);
-      L.a[r][n] = silu(c[i]);
-      if (z2s) z2s[r * 64 + n] = c[i];
-    }
-  }
-  __syncthreads();
-  // layer 3: y = h2 . w3 + b3 (one output column: B has w3 in column 0)
-#pragma unroll
-  for (int ks = 0; ks < 4; ++ks) ah[ks] = lds_a(L.a, lane, ks);
-  f32x16 c;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) c[i] = W.b3[0];
-#pragma unroll
-  for (int ks = 0; ks < 4; ++ks) c = mfma(ah[ks], w_b(W.w3, WSPH_MLP_H, 1, lane, 0, ks), c);
-#pragma unroll
-  for (int i = 0; i < 16; ++i) y[i] = c[i];
-  __syncthreads();
-}
 
 // ---- parameter conversion ---------------------------------------------------
 // params (fp32): W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3
@@ -143,23 +29,13 @@ __global__ void k_to_bf16(const float *__restrict__ p, __bf16 *__restrict__ w1,
   }
 }
 
-static Weights weights_of(const MlpDev &m) {
-  Weights W;
-  W.w1 = m.w1;
-  W.w2 = m.w2;
-  W.w3 = m.w3;
-  W.b1 = m.params + WSPH_MLP_OFF_B1;
-  W.b2 = m.params + WSPH_MLP_OFF_B2;
-  W.b3 = m.params + WSPH_MLP_OFF_B3;
-  return W;
-}
 
 // ---- inference on a feature matrix -------------------------------------------
 constexpr int kMlpWaves = 4;
 
 __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_forward(Weights W, const float *__restrict__ x,
                                                                 int n, float *__restrict__ y) {
-  __shared__ WaveLds lds[kMlpWaves];
+  __shared__ float act[kMlpWaves][32][65];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tiles = (n + 31) / 32;
   // every wave of the block runs the same number of loop trips (the tile
@@ -170,7 +46,7 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_forward(Weights W, const
     const int tt = live ? t : tiles - 1;
     const int rows = live ? min(32, n - 32 * tt) : 0;
     float out[16];
-    forward_tile(W, x + (long)32 * tt * WSPH_MLP_IN, rows, lds[wave], nullptr, nullptr, out);
+    forward_tile(W, x + (long)32 * tt * WSPH_MLP_IN, rows, act[wave], nullptr, nullptr, out);
     if (live && (lane & 31) == 0)
       for (int i = 0; i < 16; ++i) {
         const int r = drow(lane, i);
@@ -209,7 +85,7 @@ __global__ __launch_bounds__(64 * kBwdWaves) void k_mlp_backward(Weights W, cons
     const int rows = live ? min(32, n - 32 * tile) : 0;
     const float *xt = x + (long)32 * tile * WSPH_MLP_IN;
     float y[16];
-    forward_tile(W, xt, live ? min(32, n - 32 * tile) : 0, L, z1s[wave], z2s[wave], y);
+    forward_tile(W, xt, rows, L.a, z1s[wave], z2s[wave], y);
     // dL/dy = (y - t) / B on the lanes that hold column 0
     if ((lane & 31) == 0) {
       for (int i = 0; i < 16; ++i) {

@@ -339,10 +339,15 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
   float pv;
   if (start_mode == WSPH_WARM_ZERO)
     pv = update_p(0.0f, rho_adv, 0.0f, a_ii, c);  // A 0 = 0: the first Jacobi update
-  else if (start_mode == WSPH_WARM_PREVIOUS)
-    pv = b.pprev[i];
+  else if (start_mode == WSPH_WARM_PREVIOUS || start_mode == WSPH_WARM_LEARNED)
+    pv = b.pprev[i];  // the learned start overwrites it in the feature pass
   else
     pv = b.pstart[i];
+  if (b.seed)  // self features of the pressure predictor
+    b.seed[i] = make_float4(clamp_feat((rho_adv / c.rho0 - 1.0f) * 100.0f),
+                            clamp_feat(a_ii * c.feat_k1 / (dt * dt)),
+                            clamp_feat((float)nf * (1.0f / 32.0f)),
+                            clamp_feat(b.pprev[i] * c.inv_pref));
   b.dp[0][i] = make_float4(dix, diy, diz, pv);
   b.sap[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   b.ap[i] = 0.0f;
@@ -584,9 +589,9 @@ __device__ __forceinline__ float from_ordered_bits(unsigned u) {
 __global__ void k_integrate(Bufs b, SphConst c, int src) {
   __shared__ unsigned s_vmax[16];
   __shared__ int s_bad[16];
-  __shared__ double s_ke[16], s_y[16];
+  __shared__ double s_ke[16], s_y[16], s_en[16], s_ec[16];
   __shared__ unsigned s_ymax[16];
-  double ke = 0.0, ysum = 0.0;
+  double ke = 0.0, ysum = 0.0, en = 0.0, ec = 0.0;
   unsigned ymax = 0u;
   if (src < 0) src = b.st->final_src;
   const int i = particle_index();
@@ -609,7 +614,22 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
     b.pos[i] = make_float4(x, y, z, p.w);
     b.vel[i] = make_float4(vx, vy, vz, 0.0f);
     b.pfinal[i] = di.w;
-    const float a = b.aux[i].y;
+    const float4 au = b.aux[i];
+    const float a = au.y;
+    if (b.target || b.judge) {
+      const float carry = b.pprev[i];  // this step's carried start value
+      if (b.target) b.target[i] = clamp_feat((di.w - carry) * c.inv_pref);
+      if (b.judge) {
+        // start errors weighted by min(|a_ii / a_ref|, 1): a particle's share
+        // of the density residual scales with a_ii
+        float wgt = fabsf(a * c.feat_k1 / (dt * dt));
+        wgt = wgt < 1.0f ? wgt : 1.0f;
+        const double e1 = (double)wgt * ((double)au.z - (double)di.w);
+        const double e2 = (double)wgt * ((double)carry - (double)di.w);
+        en = e1 * e1;
+        ec = e2 * e2;
+      }
+    }
     b.pprev[i] = (a > -c.inv_small && a < c.inv_small) ? 0.0f : di.w;
     bad = !(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(l));
     if (bad) vbits = 0;
@@ -623,6 +643,10 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
   for (int o = 32; o > 0; o >>= 1) {
     ke += __shfl_xor(ke, o, 64);
     ysum += __shfl_xor(ysum, o, 64);
+    if (b.judge) {
+      en += __shfl_xor(en, o, 64);
+      ec += __shfl_xor(ec, o, 64);
+    }
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) {
@@ -631,23 +655,31 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
     s_ke[wid] = ke;
     s_y[wid] = ysum;
     s_ymax[wid] = ymax;
+    s_en[wid] = en;
+    s_ec[wid] = ec;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned m = 0, my = 0;
     int bb = 0;
-    double k2 = 0.0, y2 = 0.0;
+    double k2 = 0.0, y2 = 0.0, n2 = 0.0, c2 = 0.0;
     for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
       m = max(m, s_vmax[w]);
       my = max(my, s_ymax[w]);
       bb |= s_bad[w];
       k2 += s_ke[w];
       y2 += s_y[w];
+      n2 += s_en[w];
+      c2 += s_ec[w];
     }
     b.vmax_part[blockIdx.x] = m;
     b.ymax_part[blockIdx.x] = my;
     b.ke_part[blockIdx.x] = k2;
     b.y_part[blockIdx.x] = y2;
+    if (b.judge) {
+      b.err_part[2 * blockIdx.x] = n2;
+      b.err_part[2 * blockIdx.x + 1] = c2;
+    }
     if (bb) atomicOr(&b.st->status, (int)ST_NAN);
   }
 }
@@ -655,11 +687,11 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
 // max |v| and the step statistics from the per-block partials of
 // k_integrate (summed in block order: deterministic), appended to the
 // history ring.
-__global__ void k_vmax_reduce(Bufs b, int nparts) {
+__global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode) {
   __shared__ unsigned s[16], sy[16];
-  __shared__ double sk[16], ss[16];
+  __shared__ double sk[16], ss[16], sn[16], sc[16];
   unsigned m = 0, my = 0;
-  double ke = 0.0, ys = 0.0;
+  double ke = 0.0, ys = 0.0, en = 0.0, ec = 0.0;
   // fixed assignment of partials to threads and a fixed combining tree:
   // the same sum on every run
   for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
@@ -667,18 +699,26 @@ __global__ void k_vmax_reduce(Bufs b, int nparts) {
     my = max(my, b.ymax_part[k]);
     ke += b.ke_part[k];
     ys += b.y_part[k];
+    if (b.judge) {
+      en += b.err_part[2 * k];
+      ec += b.err_part[2 * k + 1];
+    }
   }
   m = wave_max_u32(m);
   my = wave_max_u32(my);
   for (int o = 32; o > 0; o >>= 1) {
     ke += __shfl_xor(ke, o, 64);
     ys += __shfl_xor(ys, o, 64);
+    en += __shfl_xor(en, o, 64);
+    ec += __shfl_xor(ec, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
     s[threadIdx.x >> 6] = m;
     sy[threadIdx.x >> 6] = my;
     sk[threadIdx.x >> 6] = ke;
     ss[threadIdx.x >> 6] = ys;
+    sn[threadIdx.x >> 6] = en;
+    sc[threadIdx.x >> 6] = ec;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -686,11 +726,15 @@ __global__ void k_vmax_reduce(Bufs b, int nparts) {
     my = 0;
     ke = 0.0;
     ys = 0.0;
+    en = 0.0;
+    ec = 0.0;
     for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
       m = max(m, s[w]);
       my = max(my, sy[w]);
       ke += sk[w];
       ys += ss[w];
+      en += sn[w];
+      ec += sc[w];
     }
     DevState *st = b.st;
     st->vmax_bits = m;
@@ -703,6 +747,19 @@ __global__ void k_vmax_reduce(Bufs b, int nparts) {
     r.v_max = __uint_as_float(m);
     r.iterations = st->iterations;
     r.converged = st->converged;
+    r.start_source = start_mode;
+    r.err_net = 0.0;
+    r.err_carry = 0.0;
+    if (start_mode == WSPH_WARM_LEARNED) {
+      // net_off still holds the choice this step's feature pass made
+      if (st->net_off) r.start_source = WSPH_WARM_PREVIOUS;
+      if (b.judge && b.n > 0) {
+        r.err_net = sqrt(en / b.n);
+        r.err_carry = sqrt(ec / b.n);
+        // the carried pressure judges the network for the next step
+        st->net_off = en > ec ? 1 : 0;
+      }
+    }
     st->step_count += 1;
   }
 }
@@ -902,10 +959,11 @@ void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P,
   }
 }
 
-void launch_integrate(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s) {
+void launch_integrate(const Bufs &b, const SphConst &c, int src, int start_mode, int block,
+                      hipStream_t s) {
   if (b.n == 0) return;
   k_integrate<<<nblocks(b.n, block), block, 0, s>>>(b, c, src);
-  k_vmax_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block));
+  k_vmax_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block), start_mode);
 }
 
 void launch_clock(const Bufs &b, const wsph_params &P, hipStream_t s) {

@@ -54,6 +54,10 @@ class PressureNet:
     """Owns a ``wsph_mlp`` context: parameters, BF16 copies, Adam moments."""
 
     def __init__(self, max_batch: int = 65536, params=None, seed: int = 0):
+        # torch brings up the GPU runtime first: the tensors handed to the
+        # network are torch's
+        if not _torch().cuda.is_available():
+            raise RuntimeError("PressureNet needs a ROCm GPU (torch.cuda); there is no CPU path")
         self._lib = N.load()
         assert self._lib.wsph_mlp_nparams() == N_PARAM
         h = C.c_void_p()

@@ -10,7 +10,7 @@ used by the benchmark workloads in the same order and precision.
 from __future__ import annotations
 
 from dataclasses import dataclass
-from typing import Optional, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -103,6 +103,43 @@ def dam_break(cfg: SolverConfig) -> Scene:
     xb32 = _c32(xb)
     lo, hi = _bounds(x32, xb32)
     return Scene(x32, xb32, np.zeros_like(xb32), xb32.shape[0], lo, hi)
+
+
+def batch_configs(cfg: SolverConfig) -> List[SolverConfig]:
+    """Configurations of the independent dam-break scenes of a batch.
+
+    From ``default_rng(cfg.seed)``, per scene: a fluid count
+    exp(U(ln n_min, ln n_max)); box edges U(0.5, 2) m; a fluid column against
+    the box corner whose x and z extents are U(0.15, 0.6) of the box, its
+    height set by the count (capped at 0.9 of the box, the footprint then
+    widened); +-1% jitter with a per-scene seed. Spacing and physics are
+    those of ``cfg``.
+    """
+    ex = cfg.extra
+    ns, n_min, n_max = int(ex.get("n_scenes", 256)), ex.get("n_min", 2000), ex.get("n_max", 64000)
+    rng = np.random.default_rng(cfg.seed)
+    d = cfg.spacing
+    out = []
+    for k in range(ns):
+        n = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        box = tuple(float(v) for v in rng.uniform(0.5, 2.0, 3))
+        fx, fz = rng.uniform(0.15, 0.6, 2)
+        # the column starts one spacing inside the walls; keep one spacing free
+        room = [int((b - 2 * d) / d) for b in box]
+        cx = max(2, min(room[0], int(round(fx * box[0] / d))))
+        cz = max(2, min(room[2], int(round(fz * box[2] / d))))
+        cy_max = max(2, int(0.9 * box[1] / d))
+        cy = int(round(n / (cx * cz)))
+        if cy > cy_max:
+            grow = np.sqrt(n / (cx * cy_max * cz))
+            cx = max(2, min(room[0], int(round(cx * grow))))
+            cz = max(2, min(room[2], int(round(cz * grow))))
+            cy = cy_max
+        cy = max(2, min(cy, cy_max))
+        out.append(cfg.replace(scene="dam_break", box=box, fluid_cells=(cx, cy, cz),
+                               fluid_origin=(d, d, d), jitter=0.01, seed=cfg.seed * 1000 + k,
+                               boundary_layers=1, extra={}))
+    return out
 
 
 GENERATORS = {

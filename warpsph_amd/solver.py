@@ -42,7 +42,7 @@ class IISPH:
     def __init__(self, stage_path: Optional[str] = "example_sph.usd", preview: bool = False,
                  verbose: bool = False, config: Optional[cfgmod.SolverConfig] = None,
                  device: Optional[str] = None, scene: Optional[scn.Scene] = None,
-                 collect_stats: bool = False):
+                 collect_stats: bool = False, predictor=None):
         torch = _torch()
         self.verbose = verbose
         self.collect_stats = collect_stats
@@ -84,6 +84,9 @@ class IISPH:
         x0 = torch.from_numpy(np.ascontiguousarray(sc.x, np.float32)).to(self.device)
         _sync_torch()
         N.check("wsph_set_fluid", self._lib.wsph_set_fluid(h, x0.data_ptr(), None, self.n))
+        self.predictor = None
+        if predictor is not None or cfg.warm_start == cfgmod.WARM_LEARNED:
+            self.set_predictor(predictor)
         self.last_stats: Optional[dict] = None
         self._stats = N.WsphStepStats()
         if verbose:
@@ -161,8 +164,31 @@ class IISPH:
         cnt = C.c_int()
         N.check("wsph_get_history", self._lib.wsph_get_history(self._h, recs, max_records,
                                                                C.byref(cnt)))
-        keys = [k for k, _ in N.WsphStepRecord._fields_ if k != "pad"]
+        keys = [k for k, _ in N.WsphStepRecord._fields_]
         return [{k: getattr(recs[i], k) for k in keys} for i in range(cnt.value)]
+
+    # ---- learned start ------------------------------------------------------
+    def set_predictor(self, net) -> None:
+        """Attach a :class:`warpsph_amd.mlp.PressureNet` for the learned start
+        mode (the solver keeps a reference; None detaches)."""
+        if net is None and self.cfg.warm_start == cfgmod.WARM_LEARNED:
+            raise ValueError("the learned start mode needs a predictor network")
+        self.predictor = net
+        N.check("wsph_set_predictor", self._lib.wsph_set_predictor(
+            self._h, None if net is None else net.handle))
+
+    def set_pair_logging(self, on: bool = True) -> None:
+        """Keep each step's (features, target) rows for :meth:`pairs`."""
+        N.check("wsph_set_pair_logging", self._lib.wsph_set_pair_logging(self._h, int(bool(on))))
+
+    def pairs(self):
+        """(features [n, 8], targets [n]) of the last step, as device tensors."""
+        torch = _torch()
+        f = torch.empty((self.n, 8), dtype=torch.float32, device=self.device)
+        t = torch.empty(self.n, dtype=torch.float32, device=self.device)
+        _sync_torch()
+        N.check("wsph_get_pairs", self._lib.wsph_get_pairs(self._h, f.data_ptr(), t.data_ptr()))
+        return f, t
 
     def set_pressure(self, p) -> None:
         """Caller-supplied start pressure for the next step (original order)."""
