@@ -10,8 +10,14 @@ synchronisations with the particle state resident in HBM. ``value`` counts
 fluid particle-steps per second over the steps whose pressure solve
 converged (residual <= eta).
 
-The roofline figure covers the dominant kernel, the persistent Jacobi loop:
-its algorithmic bytes are 92 B per fluid particle per A p evaluation
+The headline run uses the learned pressure start (the predictor network in
+warpsph_amd/pressure_net.npy, trained by warpsph_amd.train_predictor); the
+same K steps are then run from fresh state with the plain starts, the
+previous step's pressure and p = 0 (the reference's), and their throughput
+and Jacobi iterations per step are reported beside it.
+
+The roofline figure covers the dominant kernel pair, J1 + J2 of the Jacobi
+loop: its algorithmic bytes are 92 B per fluid particle per A p evaluation
 (J1: x, p, rho, d_ii read 32 B, s and A p partial written 16 B; J2: x, s,
 A p, a_ii, p, rho_adv read 40 B, p_next written 4 B), and its duration is
 measured with HIP events on the solver's stream over the timed steps.
@@ -34,6 +40,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+WARM = {}
 BYTES_PER_AP_EVAL = 92  # per fluid particle, J1 48 + J2 44
 BYTES_PER_STEP_FIXED = 264  # grid 84 + density 16 + v_adv 40 + assembly 48 + integrate 76
 
@@ -46,7 +53,11 @@ def parse():
     ap.add_argument("--preset", default="dambreak_1m")
     ap.add_argument("--jacobi", choices=["persistent", "host", "device"], default="device")
     ap.add_argument("--chunk", type=int, default=0, help="device-loop iterations per chunk")
-    ap.add_argument("--warm-start", choices=["zero", "previous"], default="previous")
+    ap.add_argument("--warm-start", choices=["zero", "previous", "learned"], default="learned")
+    ap.add_argument("--net", default=str(ROOT / "warpsph_amd" / "pressure_net.npy"),
+                    help="predictor parameters for the learned start")
+    ap.add_argument("--compare", default="previous,zero",
+                    help="plain starts to run after the headline run (comma list, '' for none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--layout", type=int, default=-1, help="cell layout: 0 linear, 1 Morton")
@@ -113,41 +124,20 @@ def cfg_n(x):
     return int(x.shape[0])
 
 
-def main():
-    args = parse()
-    if args.calibrate_copy_mib:
-        calibrate_copy(args.calibrate_copy_mib)
-        return
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
-        sys.exit(2)
+def timed_run(cfg, net, args, trace: bool) -> dict:
+    """W untimed steps, then K steps timed between device synchronisations."""
     import torch
-    from warpsph_amd import config
     from warpsph_amd import _native as N
     from warpsph_amd.solver import IISPH
-
-    cfg = config.preset(args.preset)
-    cfg = cfg.replace(jacobi_mode={"persistent": config.JACOBI_PERSISTENT,
-                                   "host": config.JACOBI_HOST,
-                                   "device": config.JACOBI_DEVICE}[args.jacobi],
-                      chunk_iters=args.chunk,
-                      warm_start=config.WARM_ZERO if args.warm_start == "zero"
-                      else config.WARM_PREVIOUS)
-    if args.layout >= 0:
-        cfg = cfg.replace(cell_layout=args.layout)
-    if args.coop_per_cu:
-        cfg = cfg.replace(coop_blocks_per_cu=args.coop_per_cu)
     t_setup = time.perf_counter()
-    s = IISPH(stage_path=None, config=cfg)
+    s = IISPH(stage_path=None, config=cfg, predictor=net)
     lib, h = s._lib, s._h
-    print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s n={s.n} nb={s.boundary_n}",
-          file=sys.stderr, flush=True)
-
-    for k in range(args.trace_steps):
+    print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s n={s.n} nb={s.boundary_n} "
+          f"start={cfg.warm_start}", file=sys.stderr, flush=True)
+    for k in range(args.trace_steps if trace else 0):
         st = N.WsphStepStats()
         t = time.perf_counter()
         rc = lib.wsph_step(h, C.byref(st))
-        tm1 = (C.c_double * 8)()
         print(f"[bench] trace step {k}: rc={rc} {1e3 * (time.perf_counter() - t):.2f} ms "
               f"iters={st.iterations} residual={st.residual:.3e} vmax={st.v_max:.3f} "
               f"status={st.status} max_ff={st.max_ff_count} max_fs={st.max_fs_count} "
@@ -171,20 +161,54 @@ def main():
     N.check("wsph_last_timings", lib.wsph_last_timings(h, tm, 8))
     if c1["status"]:
         raise RuntimeError(f"device status {c1['status']} during the timed run")
-    elapsed = t1 - t0
     steps = c1["steps"] - c0["steps"]
-    iters = c1["iterations"] - c0["iterations"]
     nonconv = c1["nonconverged"] - c0["nonconverged"]
-    ap_evals = c1["ap_evals"] - c0["ap_evals"]
-    n = s.n
-    value = n * (steps - nonconv) / elapsed
+    elapsed = t1 - t0
+    return dict(solver=s, n=s.n, elapsed=elapsed, steps=steps, tm=tm,
+                iters_per_step=(c1["iterations"] - c0["iterations"]) / max(1, steps),
+                nonconverged=nonconv, ap_evals=c1["ap_evals"] - c0["ap_evals"],
+                value=s.n * (steps - nonconv) / elapsed)
+
+
+def main():
+    args = parse()
+    if args.calibrate_copy_mib:
+        calibrate_copy(args.calibrate_copy_mib)
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    from warpsph_amd import config
+    WARM.update(zero=config.WARM_ZERO, previous=config.WARM_PREVIOUS,
+                learned=config.WARM_LEARNED)
+
+    base = config.preset(args.preset)
+    base = base.replace(jacobi_mode={"persistent": config.JACOBI_PERSISTENT,
+                                     "host": config.JACOBI_HOST,
+                                     "device": config.JACOBI_DEVICE}[args.jacobi],
+                        chunk_iters=args.chunk)
+    if args.layout >= 0:
+        base = base.replace(cell_layout=args.layout)
+    if args.coop_per_cu:
+        base = base.replace(coop_blocks_per_cu=args.coop_per_cu)
+    net = None
+    if args.warm_start == "learned":
+        from warpsph_amd.mlp import PressureNet
+        if not Path(args.net).exists():
+            raise SystemExit(f"{args.net} is missing: train it with "
+                             "`python -m warpsph_amd.train_predictor --out {args.net}`")
+        net = PressureNet.load(args.net, max_batch=1024)
+    cfg = base.replace(warm_start=WARM[args.warm_start])
+    r = timed_run(cfg, net, args, trace=True)
+    s, n, elapsed, steps, tm = r["solver"], r["n"], r["elapsed"], r["steps"], r["tm"]
+    ap_evals = r["ap_evals"]
     jac_ms = tm[2] / max(1, int(tm[6]))
     jac_bytes = n * BYTES_PER_AP_EVAL * ap_evals / max(1, steps)
     achieved = jac_bytes / (jac_ms * 1e-3) / 1e9
     workload = args.preset
     line = {
-        "metric": "fluid particle-steps/s @1M, rho-err<=0.1%",
-        "value": value,
+        "metric": "fluid particle-steps/s @1M, rho-err<=0.1%; Jacobi iters/step learned vs plain",
+        "value": r["value"],
         "unit": "fluid particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -198,8 +222,8 @@ def main():
         "config": {"workload": workload, "fluid_particles": n, "boundary_particles": s.boundary_n,
                    "spacing": cfg.spacing, "dt": cfg.dt0, "jacobi": args.jacobi,
                    "warm_start": args.warm_start, "eta": cfg.eta},
-        "jacobi_iters_per_step": iters / max(1, steps),
-        "nonconverged_steps": nonconv,
+        "jacobi_iters_per_step": r["iters_per_step"],
+        "nonconverged_steps": r["nonconverged"],
         "phase_ms_per_step": {"neighbor_search": tm[0] / max(1, tm[6]),
                               "predict_advection": tm[1] / max(1, tm[6]),
                               "pressure_solve": jac_ms,
@@ -213,10 +237,32 @@ def main():
         "step_roofline_frac": (n * (BYTES_PER_STEP_FIXED * steps + BYTES_PER_AP_EVAL * ap_evals)
                                / elapsed / 1e9) / HBM_PEAK_GBS,
     }
+    if net is not None:
+        h = s.history(args.steps)
+        line["learned"] = {
+            "steps_on_network": sum(1 for q in h if q["start_source"] == config.WARM_LEARNED),
+            "start_err_learned_pa": sum(q["start_err_learned"] for q in h) / max(1, len(h)),
+            "start_err_carry_pa": sum(q["start_err_carry"] for q in h) / max(1, len(h)),
+            "params": str(Path(args.net).relative_to(ROOT)) if Path(args.net).is_relative_to(ROOT)
+            else args.net}
+    x_end = s.x.cpu().numpy()
+    v_end = s.v.cpu().numpy()
+    s.close()
+    plain = {}
+    for name in [c for c in args.compare.split(",") if c and c != args.warm_start]:
+        q = timed_run(base.replace(warm_start=WARM[name]), None, args, trace=False)
+        plain[name] = {"value": q["value"], "ms_per_step": q["elapsed"] / args.steps * 1e3,
+                       "jacobi_iters_per_step": q["iters_per_step"],
+                       "nonconverged_steps": q["nonconverged"]}
+        q["solver"].close()
+        print(f"[bench] {name} start: {plain[name]}", file=sys.stderr, flush=True)
+    if plain:
+        line["plain_starts"] = plain
+        if "previous" in plain:
+            line["speedup_vs_previous_start"] = r["value"] / plain["previous"]["value"]
     if not args.no_cpu_baseline:
-        x = s.x.cpu().numpy()
-        v = s.v.cpu().numpy()
-        line["cpu_baseline"] = cpu_baseline(cfg, x, v, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(base.replace(warm_start=config.WARM_PREVIOUS), x_end,
+                                            v_end, args.cpu_seconds)
         line["cpu_baseline"]["host_cpu"] = _cpu_model()
     print(json.dumps(line), flush=True)
 
