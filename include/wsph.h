@@ -236,6 +236,22 @@ int wsph_get_counters(wsph_solver *s, long long *out, int nout);
  * kept on the device so that long runs need no per-step synchronisation. */
 int wsph_get_history(wsph_solver *s, wsph_step_record *out, int max_records, int *count);
 
+/* ---- batched scenes (no reference counterpart) -------------------------
+ * Many independent scenes solved together: call after wsph_create and before
+ * the particle calls (wsph_set_boundary / wsph_set_fluid), whose
+ * arrays then hold the scenes' particles concatenated in scene order. Each
+ * scene keeps its own fluid and boundary hash grids (dims per scene, 3 ints
+ * each), so particles of different scenes never interact and every scene
+ * evolves exactly as it would alone. Each scene runs its own Jacobi stop
+ * test (IISPH.py:1200-1207) and stops iterating when it has converged while
+ * the others continue. Requires WSPH_DT_FIXED (one dt for all scenes) and
+ * the device or persistent Jacobi driver; params.grid_dims are unused. */
+int wsph_set_scenes(wsph_solver *s, int nscenes, const int *fluid_counts,
+                    const int *boundary_counts, const int *grid_dims, const int *bgrid_dims);
+/* Per-scene results of the last step (host arrays of nscenes entries, each
+ * optional): Jacobi iterations, residual <= eta, residual at exit. */
+int wsph_get_scene_stats(wsph_solver *s, int *iterations, int *converged, double *residual);
+
 /* ---- learned pressure start value (no reference counterpart) -----------
  * Per-particle network 8 -> 64 -> 64 -> 1 with SiLU, evaluated on the
  * matrix cores with BF16 operands and FP32 accumulation. Parameters are a

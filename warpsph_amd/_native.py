@@ -128,6 +128,8 @@ SIGNATURES = [
     ("wsph_mlp_get_params", _I, [_P, _P]),
     ("wsph_mlp_forward", _I, [_P, _P, _I, _P]),
     ("wsph_mlp_train_step", _I, [_P, _P, _P, _I, C.c_float, C.POINTER(C.c_float), _P]),
+    ("wsph_set_scenes", _I, [_P, _I, _P, _P, _P, _P]),
+    ("wsph_get_scene_stats", _I, [_P, _P, _P, _P]),
     ("wsph_set_predictor", _I, [_P, _P]),
     ("wsph_set_pair_logging", _I, [_P, _I]),
     ("wsph_get_pairs", _I, [_P, _P, _P]),

@@ -176,6 +176,17 @@ struct wsph_solver {
   int *done_flags = nullptr;
   hipEvent_t chunk_ev[2] = {};
   int step_count = 0;
+  // batched scenes (nscenes == 0: one scene described by the params)
+  int nscenes = 0;
+  std::vector<int> scene_nf, scene_nb;
+  std::vector<SceneGrid> scene_grid;
+  SceneGrid *d_scenes = nullptr;
+  int *sid[2] = {nullptr, nullptr};
+  int *bsid = nullptr;
+  unsigned long long *sc_sum = nullptr;
+  unsigned *sc_cnt = nullptr;
+  int *sc_done = nullptr, *sc_iters = nullptr, *sc_final = nullptr;
+  double *sc_res = nullptr;
   // learned start and training-pair logging
   wsph_mlp *net = nullptr;
   bool log_pairs = false, have_pairs = false;
@@ -207,13 +218,13 @@ static long grid_cells(const int *dims, int layout) {
   return (long)dims[0] * dims[1] * dims[2];
 }
 
-static int alloc_grid(GridBufs &g, const int *dims, int layout) {
+static int alloc_grid(GridBufs &g, const int *dims, int layout, long ncell = -1) {
   for (auto p : g.scan) dfree(p);
   g.scan.clear();
   dfree(g.count);
   dfree(g.start);
   memcpy(g.dims, dims, sizeof(g.dims));
-  g.ncell = grid_cells(dims, layout);
+  g.ncell = ncell >= 0 ? ncell : grid_cells(dims, layout);
   HIP_TRY(dalloc(&g.count, g.ncell));
   HIP_TRY(dalloc(&g.start, g.ncell));
   long sizes[8];
@@ -276,6 +287,18 @@ static Bufs make_bufs(wsph_solver *s) {
   }
   b.err_part = s->err_part;
   b.judge = learned && s->net && s->err_part;
+  if (s->nscenes > 0) {
+    b.scenes = s->d_scenes;
+    b.sid = s->sid[s->cur];
+    b.bsid = s->bsid;
+    b.nscenes = s->nscenes;
+    b.sc_sum = s->sc_sum;
+    b.sc_cnt = s->sc_cnt;
+    b.sc_done = s->sc_done;
+    b.sc_iters = s->sc_iters;
+    b.sc_final = s->sc_final;
+    b.sc_res = s->sc_res;
+  }
   return b;
 }
 
@@ -418,6 +441,8 @@ static void free_fluid(wsph_solver *s) {
   dfree(s->cnt_ff);
   dfree(s->nbr_fs);
   dfree(s->cnt_fs);
+  dfree(s->sid[0]);
+  dfree(s->sid[1]);
   dfree(s->seed);
   dfree(s->feat);
   dfree(s->target);
@@ -445,6 +470,14 @@ int wsph_destroy(wsph_solver *s) {
   dfree(s->bpos);
   dfree(s->bvel);
   dfree(s->borig);
+  dfree(s->bsid);
+  dfree(s->d_scenes);
+  dfree(s->sc_sum);
+  dfree(s->sc_cnt);
+  dfree(s->sc_done);
+  dfree(s->sc_iters);
+  dfree(s->sc_final);
+  dfree(s->sc_res);
   dfree(s->wt);
   dfree(s->gt);
   dfree(s->st);
@@ -489,8 +522,12 @@ int wsph_set_tables(wsph_solver *s, const float *w_table, int w_size, const floa
 
 // Sort points (float4, current order = input order) into cell order.
 static int sort_points(wsph_solver *s, GridBufs &g, const float4 *pos, const int *orig, int n,
-                       int *slot, int *rank, int *tmp, int *perm) {
+                       int *slot, int *rank, int *tmp, int *perm, const int *sid = nullptr,
+                       int boundary = 0) {
   GridArgs a;
+  a.sid = s->nscenes > 0 ? sid : nullptr;
+  a.scenes = s->nscenes > 0 ? s->d_scenes : nullptr;
+  a.boundary = boundary;
   a.n = n;
   a.ncell = g.ncell;
   a.inv = s->C.cell_inv;
@@ -521,8 +558,23 @@ int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) 
   HIP_TRY(dalloc(&s->bvel, nb));
   HIP_TRY(dalloc(&s->borig, nb));
   float4 *p4 = nullptr, *v4 = nullptr;
-  int *id = nullptr, *sl = nullptr, *rk = nullptr, *tp = nullptr, *pm = nullptr;
+  int *id = nullptr, *sl = nullptr, *rk = nullptr, *tp = nullptr, *pm = nullptr, *bs = nullptr;
   int rc = WSPH_OK;
+  if (s->nscenes > 0) {
+    long total = 0;
+    for (int v : s->scene_nb) total += v;
+    if (total != nb) return fail(WSPH_E_ARG, "boundary count %d differs from the scenes' %ld", nb, total);
+    dfree(s->bsid);
+    HIP_TRY(dalloc(&s->bsid, nb));
+    if (nb > 0) {
+      std::vector<int> h(nb);
+      int o = 0;
+      for (int k = 0; k < s->nscenes; ++k)
+        for (int q = 0; q < s->scene_nb[k]; ++q) h[o++] = k;
+      HIP_TRY(dalloc(&bs, nb));
+      HIP_TRY(hipMemcpy(bs, h.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+    }
+  }
   if (nb > 0) {
     HIP_TRY(dalloc(&p4, nb));
     HIP_TRY(dalloc(&v4, nb));
@@ -535,10 +587,11 @@ int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) 
     k_pack3<<<nblk(nb, 256), 256, 0, s->stream>>>(vb, nb, v4);
     k_iota<<<nblk(nb, 256), 256, 0, s->stream>>>(id, nb);
   }
-  rc = sort_points(s, s->bg, p4, id, nb, sl, rk, tp, pm);
+  rc = sort_points(s, s->bg, p4, id, nb, sl, rk, tp, pm, bs, 1);
   if (!rc && nb > 0) {
     k_gather_boundary<<<nblk(nb, 256), 256, 0, s->stream>>>(pm, nb, p4, v4, s->bpos, s->bvel,
                                                             s->borig);
+    if (bs) gather_i32(pm, nb, bs, s->bsid, s->stream);
     Bufs b = make_bufs(s);
     launch_boundary_psi(b, s->C, s->stream);
     if (hipGetLastError() != hipSuccess) rc = fail(WSPH_E_HIP, "boundary psi launch failed");
@@ -551,6 +604,7 @@ int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) 
   dfree(rk);
   dfree(tp);
   dfree(pm);
+  dfree(bs);
   if (e != hipSuccess) return fail(WSPH_E_HIP, "boundary setup: %s", hipGetErrorString(e));
   return rc;
 }
@@ -559,6 +613,11 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
   if (!s || n < 0 || (n > 0 && !x)) return fail(WSPH_E_ARG, "bad fluid arguments");
   if (!s->wt) return fail(WSPH_E_ARG, "call wsph_set_tables first");
   if (!s->bpos) return fail(WSPH_E_ARG, "call wsph_set_boundary first");
+  if (s->nscenes > 0) {
+    long total = 0;
+    for (int v : s->scene_nf) total += v;
+    if (total != n) return fail(WSPH_E_ARG, "fluid count %d differs from the scenes' %ld", n, total);
+  }
   if (n != s->cap_n) {
     free_fluid(s);
     for (int k = 0; k < 2; ++k) {
@@ -590,6 +649,17 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
     HIP_TRY(dalloc(&s->nbr_ff, (size_t)n * s->P.max_ff));
     HIP_TRY(dalloc(&s->nbr_fs, (size_t)n * s->P.max_fs));
     s->cap_n = n;
+  }
+  if (s->nscenes > 0 && n > 0) {
+    dfree(s->sid[0]);
+    dfree(s->sid[1]);
+    HIP_TRY(dalloc(&s->sid[0], n));
+    HIP_TRY(dalloc(&s->sid[1], n));
+    std::vector<int> h(n);
+    int o = 0;
+    for (int k = 0; k < s->nscenes; ++k)
+      for (int q = 0; q < s->scene_nf[k]; ++q) h[o++] = k;
+    HIP_TRY(hipMemcpy(s->sid[0], h.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   }
   s->n = n;
   s->cur = 0;
@@ -642,10 +712,12 @@ int wsph_neighbor_search(wsph_solver *s) {
   s->step_count++;
   rec(s, 0);
   const int c = s->cur, o = c ^ 1;
-  int rc = sort_points(s, s->fg, s->pos[c], s->orig[c], s->n, s->slot, s->rank, s->tmp, s->perm);
+  int rc = sort_points(s, s->fg, s->pos[c], s->orig[c], s->n, s->slot, s->rank, s->tmp, s->perm,
+                       s->sid[c], 0);
   if (rc) return rc;
   gather_fluid(s->perm, s->n, s->pos[c], s->vel[c], s->pprev[c], s->orig[c], s->pos[o], s->vel[o],
                s->pprev[o], s->orig[o], s->stream);
+  if (s->nscenes > 0) gather_i32(s->perm, s->n, s->sid[c], s->sid[o], s->stream);
   s->cur = o;
   if (s->P.warm_start == WSPH_WARM_EXTERNAL && s->n > 0)
     k_gather_by_orig<<<nblk(s->n, 256), 256, 0, s->stream>>>(s->p_ext, s->orig[o], s->n,
@@ -733,8 +805,11 @@ int wsph_pressure_solve(wsph_solver *s, wsph_step_stats *stats) {
         if (__atomic_load_n(&s->done_flags[m - 1], __ATOMIC_ACQUIRE) == tag) break;
       }
     }
+    launch_batch_finalize(b, s->stream);
     s->final_dp = -1;
   } else {
+    if (s->nscenes > 0)
+      return fail(WSPH_E_UNSUPPORTED, "batched scenes need the device or persistent Jacobi driver");
     // one launch per kernel and a host convergence test each iteration,
     // as the reference's while loop (IISPH.py:1200-1296)
     const int max_it = P.force_iters > 0 ? P.force_iters : P.max_iters;
@@ -997,6 +1072,73 @@ int wsph_mlp_train_step(wsph_mlp *m, const float *x, const float *t, int n, floa
     HIP_TRY(hipMemcpyAsync(grad, m->grad, sizeof(float) * WSPH_MLP_NPARAM, hipMemcpyDeviceToHost,
                            m->stream));
   HIP_TRY(hipStreamSynchronize(m->stream));
+  return WSPH_OK;
+}
+
+int wsph_set_scenes(wsph_solver *s, int nscenes, const int *fluid_counts,
+                    const int *boundary_counts, const int *grid_dims, const int *bgrid_dims) {
+  if (!s || nscenes < 1 || nscenes > kMaxScenes || !fluid_counts || !boundary_counts ||
+      !grid_dims || !bgrid_dims)
+    return fail(WSPH_E_ARG, "bad scene arguments (1 <= nscenes <= %d)", kMaxScenes);
+  if (s->bpos || s->pos[0]) return fail(WSPH_E_ARG, "call wsph_set_scenes before setting particles");
+  if (nscenes > 1 && s->P.dt_mode != WSPH_DT_FIXED)
+    return fail(WSPH_E_UNSUPPORTED, "batched scenes share one dt: use WSPH_DT_FIXED");
+  if (nscenes > 1 && s->P.jacobi_mode == WSPH_JACOBI_HOST)
+    return fail(WSPH_E_UNSUPPORTED, "batched scenes need the device or persistent Jacobi driver");
+  std::vector<SceneGrid> g(nscenes);
+  long fb = 0, bb = 0;
+  for (int k = 0; k < nscenes; ++k) {
+    if (fluid_counts[k] < 0 || boundary_counts[k] < 0)
+      return fail(WSPH_E_ARG, "negative particle count in scene %d", k);
+    int rc;
+    if ((rc = check_dims(grid_dims + 3 * k, "scene grid")) ||
+        (rc = check_dims(bgrid_dims + 3 * k, "scene boundary grid")))
+      return rc;
+    memcpy(g[k].fdims, grid_dims + 3 * k, sizeof(g[k].fdims));
+    memcpy(g[k].bdims, bgrid_dims + 3 * k, sizeof(g[k].bdims));
+    g[k].fbase = (int)fb;
+    g[k].bbase = (int)bb;
+    fb += grid_cells(g[k].fdims, s->P.cell_layout);
+    bb += grid_cells(g[k].bdims, s->P.cell_layout);
+    if (fb > (1L << 30) || bb > (1L << 30)) return fail(WSPH_E_CAPACITY, "scene grids too large");
+  }
+  s->nscenes = nscenes;
+  s->scene_nf.assign(fluid_counts, fluid_counts + nscenes);
+  s->scene_nb.assign(boundary_counts, boundary_counts + nscenes);
+  s->scene_grid = g;
+  dfree(s->d_scenes);
+  HIP_TRY(dalloc(&s->d_scenes, nscenes));
+  HIP_TRY(hipMemcpy(s->d_scenes, g.data(), sizeof(SceneGrid) * nscenes, hipMemcpyHostToDevice));
+  for (auto p : {&s->sc_done, &s->sc_iters, &s->sc_final}) {
+    dfree(*p);
+    HIP_TRY(dalloc(p, nscenes));
+  }
+  dfree(s->sc_res);
+  HIP_TRY(dalloc(&s->sc_res, nscenes));
+  dfree(s->sc_sum);
+  dfree(s->sc_cnt);
+  HIP_TRY(dalloc(&s->sc_sum, (size_t)kMaxIters * nscenes));
+  HIP_TRY(dalloc(&s->sc_cnt, (size_t)kMaxIters * nscenes));
+  int rc = alloc_grid(s->fg, g[0].fdims, s->P.cell_layout, fb);
+  if (!rc) rc = alloc_grid(s->bg, g[0].bdims, s->P.cell_layout, bb);
+  return rc;
+}
+
+int wsph_get_scene_stats(wsph_solver *s, int *iterations, int *converged, double *residual) {
+  if (!s) return fail(WSPH_E_ARG, "null solver");
+  if (s->nscenes == 0) return fail(WSPH_E_ARG, "no batched scenes");
+  const int ns = s->nscenes;
+  std::vector<int> it(ns);
+  std::vector<double> res(ns);
+  HIP_TRY(hipMemcpyAsync(it.data(), s->sc_iters, sizeof(int) * ns, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipMemcpyAsync(res.data(), s->sc_res, sizeof(double) * ns, hipMemcpyDeviceToHost,
+                         s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < ns; ++k) {
+    if (iterations) iterations[k] = it[k];
+    if (residual) residual[k] = res[k];
+    if (converged) converged[k] = res[k] <= s->P.eta;
+  }
   return WSPH_OK;
 }
 

@@ -40,6 +40,14 @@ __host__ __device__ inline float clamp_feat(float v) {
   return v > kFeatClamp ? kFeatClamp : (v < -kFeatClamp ? -kFeatClamp : v);
 }
 
+// Batched solves: each scene has its own fluid and boundary hash grids,
+// stored one after another in the cell tables (slot = base + local slot).
+constexpr int kMaxScenes = 4096;
+struct SceneGrid {
+  int fdims[3], bdims[3];
+  int fbase, bbase;
+};
+
 // status bits accumulated on the device during a step
 enum : int {
   ST_FF_OVERFLOW = 1,

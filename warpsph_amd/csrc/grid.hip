@@ -17,17 +17,29 @@ __global__ void k_fill_u32(unsigned *p, unsigned v, long n) {
 }
 
 // Cell slot of every particle plus its arrival rank in that cell.
+// With scenes, the point's own scene grid is used and its slot offset by the
+// scene's base, so the scenes' cells never mix.
 __global__ void k_cell_slots(const float4 *__restrict__ pos, int n, float inv, int d0, int d1,
                              int d2, int layout, int *__restrict__ slot, int *__restrict__ rank,
-                             unsigned *__restrict__ count) {
+                             unsigned *__restrict__ count, const int *__restrict__ sid,
+                             const SceneGrid *__restrict__ scenes, int boundary) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 p = pos[i];
-  const int dims[3] = {d0, d1, d2};
-  const int hx = hash_axis((int)(p.x * inv), d0);
-  const int hy = hash_axis((int)(p.y * inv), d1);
-  const int hz = hash_axis((int)(p.z * inv), d2);
-  const int c = cell_slot(hx, hy, hz, dims, layout);
+  int dims[3] = {d0, d1, d2};
+  int base = 0;
+  if (scenes) {
+    const SceneGrid &g = scenes[sid[i]];
+    const int *d = boundary ? g.bdims : g.fdims;
+    dims[0] = d[0];
+    dims[1] = d[1];
+    dims[2] = d[2];
+    base = boundary ? g.bbase : g.fbase;
+  }
+  const int hx = hash_axis((int)(p.x * inv), dims[0]);
+  const int hy = hash_axis((int)(p.y * inv), dims[1]);
+  const int hz = hash_axis((int)(p.z * inv), dims[2]);
+  const int c = base + cell_slot(hx, hy, hz, dims, layout);
   slot[i] = c;
   rank[i] = (int)atomicAdd(&count[c], 1u);
 }
@@ -151,7 +163,7 @@ void sort_into_cells(const GridArgs &g, hipStream_t st) {
   k_fill_u32<<<(unsigned)((g.ncell + 255) / 256), 256, 0, st>>>(g.count, 0u, g.ncell);
   if (n > 0)
     k_cell_slots<<<nb, 256, 0, st>>>(g.pos, n, g.inv, g.dims[0], g.dims[1], g.dims[2], g.layout,
-                                     g.slot, g.rank, g.count);
+                                     g.slot, g.rank, g.count, g.sid, g.scenes, g.boundary);
   exclusive_scan(g.count, g.start, g.ncell, g.scan_bufs, st);
   if (n == 0) return;
   k_place<<<nb, 256, 0, st>>>(g.slot, g.rank, g.start, n, g.tmp);
@@ -164,6 +176,16 @@ void gather_fluid(const int *perm, int n, const float4 *pos, const float4 *vel, 
   if (n == 0) return;
   k_gather_fluid<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(perm, n, pos, vel, pprev, orig,
                                                                pos_o, vel_o, pprev_o, orig_o);
+}
+
+__global__ void k_gather_i32(const int *__restrict__ perm, int n, const int *__restrict__ src,
+                             int *__restrict__ dst) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < n) dst[f] = src[perm[f]];
+}
+
+void gather_i32(const int *perm, int n, const int *src, int *dst, hipStream_t st) {
+  if (n > 0) k_gather_i32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(perm, n, src, dst);
 }
 
 void fill_u32(unsigned *p, unsigned v, long n, hipStream_t st) {

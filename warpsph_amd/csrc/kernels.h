@@ -17,6 +17,10 @@ struct GridArgs {
   int *slot, *rank, *tmp, *perm;
   unsigned *count, *start;
   unsigned **scan_bufs;
+  // batched scenes (null: one scene with `dims`)
+  const int *sid;            // scene of each point, current order
+  const SceneGrid *scenes;
+  int boundary;              // 1: use the scenes' boundary grids
 };
 
 int scan_levels_needed(long n, long *sizes, int max_levels);
@@ -27,6 +31,7 @@ void gather_fluid(const int *perm, int n, const float4 *pos, const float4 *vel, 
                   const int *orig, float4 *pos_o, float4 *vel_o, float *pprev_o, int *orig_o,
                   hipStream_t st);
 void fill_u32(unsigned *p, unsigned v, long n, hipStream_t st);
+void gather_i32(const int *perm, int n, const int *src, int *dst, hipStream_t st);
 void linear_keys(const float *x, int n, float inv, const int *dims, int *keys, hipStream_t st);
 
 // ---- SPH passes (sph.hip) ------------------------------------------------
@@ -57,6 +62,17 @@ struct Bufs {
   float *target;        // regression targets [n] (pair logging)
   double *err_part;     // per-block (net, carry) squared start errors of k_integrate
   int judge;            // learned start: compare both start values after the solve
+  // batched scenes (scenes == null: a single scene)
+  const SceneGrid *scenes;
+  const int *sid;       // fluid scene id, sorted order
+  const int *bsid;      // boundary scene id, sorted order
+  int nscenes;
+  unsigned long long *sc_sum;  // [kMaxIters][nscenes] residual sums per iteration
+  unsigned *sc_cnt;            // [kMaxIters][nscenes] residual counts
+  int *sc_done;         // scene finished its Jacobi loop this step
+  int *sc_iters;        // iterations of the scene this step
+  int *sc_final;        // dp buffer holding the scene's final iterate
+  double *sc_res;       // residual at the scene's exit
   // fluid grid
   const unsigned *fstart, *fcount;
   // neighbour lists, column-major: list[k * n + i]
@@ -94,6 +110,9 @@ void launch_integrate(const Bufs &b, const SphConst &c, int src, int start_mode,
                       hipStream_t s);
 void launch_clock(const Bufs &b, const wsph_params &P, hipStream_t s);
 void launch_reset_state(const Bufs &b, hipStream_t s);
+// Batched scenes: after the Jacobi loop, move every scene's final iterate
+// into dp[final_src] (scenes that stopped early hold it in the other buffer).
+void launch_batch_finalize(const Bufs &b, hipStream_t s);
 // Persistent cooperative Jacobi loop; returns a hip error code.
 hipError_t launch_jacobi_persistent(const Bufs &b, const SphConst &c, const wsph_params &P,
                                     int src, int grid, int block, hipStream_t s);

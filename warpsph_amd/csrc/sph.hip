@@ -101,6 +101,50 @@ __device__ __forceinline__ void for_query_spans(float px, float py, float pz, co
   }
 }
 
+// The hash grid a query of particle i searches: the solver's grid, or the
+// grid of the particle's own scene in a batch (its cell tables start at the
+// scene's base slot).
+struct GridRef {
+  int dims[3];
+  const unsigned *start, *count;
+};
+__device__ __forceinline__ GridRef fluid_grid(const Bufs &b, const SphConst &c, int scene) {
+  GridRef g;
+  if (b.scenes) {
+    const SceneGrid &sg = b.scenes[scene];
+    g.dims[0] = sg.fdims[0];
+    g.dims[1] = sg.fdims[1];
+    g.dims[2] = sg.fdims[2];
+    g.start = b.fstart + sg.fbase;
+    g.count = b.fcount + sg.fbase;
+  } else {
+    g.dims[0] = c.fdims[0];
+    g.dims[1] = c.fdims[1];
+    g.dims[2] = c.fdims[2];
+    g.start = b.fstart;
+    g.count = b.fcount;
+  }
+  return g;
+}
+__device__ __forceinline__ GridRef boundary_grid(const Bufs &b, const SphConst &c, int scene) {
+  GridRef g;
+  if (b.scenes) {
+    const SceneGrid &sg = b.scenes[scene];
+    g.dims[0] = sg.bdims[0];
+    g.dims[1] = sg.bdims[1];
+    g.dims[2] = sg.bdims[2];
+    g.start = b.bstart + sg.bbase;
+    g.count = b.bcount + sg.bbase;
+  } else {
+    g.dims[0] = c.bdims[0];
+    g.dims[1] = c.bdims[1];
+    g.dims[2] = c.bdims[2];
+    g.start = b.bstart;
+    g.count = b.bcount;
+  }
+  return g;
+}
+
 // Candidates [s, e) tested kBatch at a time (positions loaded ahead).
 template <typename TestF>
 __device__ __forceinline__ void for_span(const float4 *__restrict__ pts, int s, int e, TestF test) {
@@ -121,7 +165,9 @@ __global__ void k_lists(Bufs b, SphConst c) {
   if (i < b.n) {
     const float4 p = b.pos[i];
     const int n = b.n;
-    for_query_spans(p.x, p.y, p.z, c, c.fdims, b.fstart, b.fcount, [&](int s0, int e0) {
+    const int scene = b.scenes ? b.sid[i] : 0;
+    const GridRef fg = fluid_grid(b, c, scene);
+    for_query_spans(p.x, p.y, p.z, c, fg.dims, fg.start, fg.count, [&](int s0, int e0) {
       for_span(b.pos, s0, e0, [&](int j, const float4 &o) {
         // distance < SMOOTHING_LENGTH and index != i (IISPH.py:72)
         if (j != i && dist2(p.x, p.y, p.z, o.x, o.y, o.z) < c.r2_lt) {
@@ -130,8 +176,9 @@ __global__ void k_lists(Bufs b, SphConst c) {
         }
       });
     });
+    const GridRef bg = boundary_grid(b, c, scene);
     if (b.nb > 0)
-      for_query_spans(p.x, p.y, p.z, c, c.bdims, b.bstart, b.bcount, [&](int s0, int e0) {
+      for_query_spans(p.x, p.y, p.z, c, bg.dims, bg.start, bg.count, [&](int s0, int e0) {
         for_span(b.bpos, s0, e0, [&](int j, const float4 &o) {
           // distance < SMOOTHING_LENGTH (IISPH.py:129)
           if (dist2(p.x, p.y, p.z, o.x, o.y, o.z) < c.r2_lt) {
@@ -160,7 +207,8 @@ __global__ void k_boundary_psi(Bufs b, SphConst c) {
   if (i >= b.nb) return;
   const float4 p = b.bpos[i];
   float rho = W_OF(T, 0.0f);
-  for_query_spans(p.x, p.y, p.z, c, c.bdims, b.bstart, b.bcount, [&](int s0, int e0) {
+  const GridRef bg = boundary_grid(b, c, b.scenes ? b.bsid[i] : 0);
+  for_query_spans(p.x, p.y, p.z, c, bg.dims, bg.start, bg.count, [&](int s0, int e0) {
     for_span(b.bpos, s0, e0, [&](int u, const float4 &o) {
       const float d2 = dist2(p.x, p.y, p.z, o.x, o.y, o.z);
       if (d2 < c.r2_lt && u != i) rho += W_OF(T, sqrtf(d2));
@@ -523,6 +571,144 @@ __global__ void k_residual_reduce(Bufs b, int nparts, int slot, int check, int s
   }
 }
 
+// ---- batched scenes: per-scene residual and stop test ----------------------
+// Particle i takes part in the Jacobi iteration unless its scene has stopped.
+__device__ __forceinline__ bool scene_active(const Bufs &b, int i, int &scene) {
+  scene = -1;
+  if (i >= b.n) return false;
+  if (!b.scenes) return true;
+  scene = b.sid[i];
+  return !b.sc_done[scene];
+}
+
+// Residual terms of the wave's particles added to their scenes' sums of
+// iteration k (integer atomics: the sums do not depend on order). Lanes are
+// grouped by scene; particles are sorted scene-major, so a wave usually
+// holds one scene and issues one atomic pair. Every lane of the wave calls
+// this; scene < 0 contributes nothing.
+__device__ __forceinline__ void scene_residual_add(const Bufs &b, int k, int scene,
+                                                   unsigned long long rs, unsigned rc) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(scene >= 0);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int sl = __shfl(scene, leader, 64);
+    const bool mine = scene == sl;
+    const unsigned long long v = wave_sum_u64(mine ? rs : 0ull);
+    const unsigned cnum = wave_sum_u32(mine ? rc : 0u);
+    if (lane == leader && cnum) {
+      atomicAdd(&b.sc_sum[(long)k * b.nscenes + sl], v);
+      atomicAdd(&b.sc_cnt[(long)k * b.nscenes + sl], cnum);
+    }
+    pending &= ~__ballot(mine);
+  }
+}
+
+// Stop test of every running scene after iteration k (IISPH.py:1200-1207 per
+// scene); `done` receives the decisions. Returns 1 if any scene continues.
+// Results are written when `write` is set.
+__device__ __forceinline__ int batch_decide(const Bufs &b, int k, int src_k, int min_it,
+                                            int max_it, int force_it, double eta,
+                                            unsigned char *done, bool write) {
+  int running = 0;
+  for (int sc = threadIdx.x; sc < b.nscenes; sc += blockDim.x) {
+    if (done[sc]) continue;
+    const unsigned long long S = __hip_atomic_load(&b.sc_sum[(long)k * b.nscenes + sc],
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned Cn = __hip_atomic_load(&b.sc_cnt[(long)k * b.nscenes + sc],
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double e;
+    if (jacobi_continue(k, S, Cn, min_it, max_it, force_it, eta, &e)) {
+      running = 1;
+    } else {
+      done[sc] = 1;
+      if (write) {
+        b.sc_iters[sc] = k + 1;
+        b.sc_res[sc] = e;
+        b.sc_final[sc] = src_k;
+      }
+    }
+  }
+  return __syncthreads_or(running);
+}
+
+// One block: the batched counterpart of k_residual_reduce's stop test.
+__global__ void k_batch_stop(Bufs b, int src0, int min_it, int max_it, int force_it, double eta) {
+  __shared__ unsigned char s_done[kMaxScenes];
+  __shared__ double s_emax[16];
+  __shared__ int s_conv[16];
+  DevState *st = b.st;
+  if (st->done) return;
+  const int k = st->iter;
+  for (int sc = threadIdx.x; sc < b.nscenes; sc += blockDim.x) s_done[sc] = (unsigned char)b.sc_done[sc];
+  __syncthreads();
+  const int any = batch_decide(b, k, src0 ^ (k & 1), min_it, max_it, force_it, eta, s_done, true);
+  for (int sc = threadIdx.x; sc < b.nscenes; sc += blockDim.x) b.sc_done[sc] = s_done[sc];
+  if (any) {
+    if (threadIdx.x == 0) st->iter = k + 1;
+    return;
+  }
+  __syncthreads();
+  // every scene has stopped: step results (largest residual, all converged)
+  double emax = 0.0;
+  int conv = 1;
+  for (int sc = threadIdx.x; sc < b.nscenes; sc += blockDim.x) {
+    const double e = b.sc_res[sc];
+    emax = e > emax ? e : emax;
+    conv &= e <= eta;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_xor(emax, o, 64);
+    emax = t > emax ? t : emax;
+    conv &= __shfl_xor(conv, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_emax[threadIdx.x >> 6] = emax;
+    s_conv[threadIdx.x >> 6] = conv;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
+      emax = s_emax[w] > emax ? s_emax[w] : emax;
+      conv &= s_conv[w];
+    }
+    st->done = 1;
+    st->iterations = k + 1;
+    st->ap_evals += k + 1;
+    st->residual = emax;
+    st->converged = conv;
+    st->final_src = src0 ^ (k & 1);
+    st->done_tag = st->step_tag;
+  }
+}
+
+// Scenes that stopped before the last iteration hold their final iterate in
+// dp[sc_final]; copy it to dp[final_src].
+__global__ void k_batch_finalize(Bufs b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  const int f = b.st->final_src;
+  const int mine = b.sc_final[b.sid[i]];
+  if (mine != f) b.dp[f][i] = b.dp[mine][i];
+}
+
+// Per-step reset of the scenes' stop state and of the residual slots of the
+// iterations the step may run.
+__global__ void k_batch_reset(Bufs b, int slots) {
+  const long total = (long)slots * b.nscenes;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long)gridDim.x * blockDim.x) {
+    b.sc_sum[q] = 0ull;
+    b.sc_cnt[q] = 0u;
+    if (q < b.nscenes) {
+      b.sc_done[q] = 0;
+      b.sc_iters[q] = 0;
+      b.sc_final[q] = 0;
+      b.sc_res[q] = 0.0;
+    }
+  }
+}
+
 template <bool kLds>
 __global__ void k_j1(Bufs b, SphConst c, int src) {
   const Tabs T = stage_tabs<kLds>(b, c);
@@ -555,7 +741,8 @@ __global__ void k_j1_dev(Bufs b, SphConst c, int src0) {
   const int i = particle_index();
   const float dt = (float)b.st->dt;
   const float t3 = c.mass * (dt * dt);
-  if (i < b.n) j1_particle(b, c, T, i, src0 ^ (k & 1), t3);
+  int scene;
+  if (scene_active(b, i, scene)) j1_particle(b, c, T, i, src0 ^ (k & 1), t3);
 }
 
 // J2 of a device-driven iteration; k_residual_reduce then runs the stop test
@@ -568,8 +755,13 @@ __global__ void k_j2_dev(Bufs b, SphConst c, int src0) {
   const int i = particle_index();
   unsigned long long rs = 0;
   unsigned rc = 0;
-  if (i < b.n) j2_particle(b, c, T, i, src0 ^ (k & 1), rs, rc);
-  block_residual_store(b, rs, rc);
+  int scene;
+  const bool act = scene_active(b, i, scene);
+  if (act) j2_particle(b, c, T, i, src0 ^ (k & 1), rs, rc);
+  if (b.scenes)
+    scene_residual_add(b, k, act ? scene : -1, rs, rc);
+  else
+    block_residual_store(b, rs, rc);
 }
 
 // ---- kick (IISPH.py:674-688) and drift (IISPH.py:691-699) ----------------
@@ -862,6 +1054,56 @@ __global__ void k_jacobi_persistent(Bufs b, SphConst c, int src, int min_it, int
   int k = 0;
   double e = 0.0;
   bool ok = true;
+  if (b.scenes) {
+    // batched scenes: every block takes the same stop decisions from the
+    // scenes' residual sums and keeps them in LDS; block 0 records them
+    __shared__ unsigned char s_done[kMaxScenes];
+    __shared__ unsigned char s_final[kMaxScenes];
+    for (int sc = threadIdx.x; sc < b.nscenes; sc += blockDim.x) s_done[sc] = 0;
+    __syncthreads();
+    for (;; ++k) {
+      for (int i0 = lo; i0 < hi; i0 += stride) {
+        const int i = i0 + threadIdx.x;
+        if (i < hi && !s_done[b.sid[i]]) j1_particle(b, c, T, i, cur, t3);
+      }
+      if (!(ok = grid_barrier(b.st))) break;
+      for (int i0 = lo; i0 < hi; i0 += stride) {
+        const int i = i0 + threadIdx.x;
+        const int sc = i < hi ? b.sid[i] : -1;
+        const bool act = sc >= 0 && !s_done[sc];
+        unsigned long long rs = 0;
+        unsigned rc = 0;
+        if (act) j2_particle(b, c, T, i, cur, rs, rc);
+        scene_residual_add(b, k, act ? sc : -1, rs, rc);
+      }
+      if (!(ok = grid_barrier(b.st))) break;
+      for (int sc = threadIdx.x; sc < b.nscenes; sc += blockDim.x)
+        if (!s_done[sc]) s_final[sc] = (unsigned char)cur;
+      __syncthreads();
+      if (!batch_decide(b, k, cur, min_it, max_it, force_it, eta, s_done, blockIdx.x == 0)) break;
+      cur ^= 1;
+    }
+    // leave every scene's final iterate in dp[0]
+    if (ok)
+      for (int i = first; i < hi; i += stride)
+        if (s_final[b.sid[i]] == 1) b.dp[0][i] = b.dp[1][i];
+    if (blockIdx.x == 0) {
+      double emax = 0.0;
+      int conv = 1;
+      for (int sc = 0; sc < b.nscenes; ++sc) {
+        const double es = b.sc_res[sc];
+        emax = es > emax ? es : emax;
+        conv &= es <= eta;
+      }
+      if (threadIdx.x == 0) {
+        b.st->iterations = k + 1;
+        b.st->ap_evals += k + 1;
+        b.st->residual = emax;
+        b.st->converged = conv;
+      }
+    }
+    return;
+  }
   for (;; ++k) {
     for (int i = first; i < hi; i += stride) j1_particle(b, c, T, i, cur, t3);
     if (!(ok = grid_barrier(b.st))) break;
@@ -954,9 +1196,17 @@ void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P,
   for (int k = 0; k < iters; ++k) {
     DISPATCH_LDS(k_j1_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0);
     DISPATCH_LDS(k_j2_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0);
-    k_residual_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block), 0, 1, src0, P.min_iters,
-                                         P.max_iters, P.force_iters, P.eta);
+    if (b.scenes)
+      k_batch_stop<<<1, 1024, 0, s>>>(b, src0, P.min_iters, P.max_iters, P.force_iters, P.eta);
+    else
+      k_residual_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block), 0, 1, src0, P.min_iters,
+                                           P.max_iters, P.force_iters, P.eta);
   }
+}
+
+void launch_batch_finalize(const Bufs &b, hipStream_t s) {
+  if (b.n == 0 || !b.scenes) return;
+  k_batch_finalize<<<nblocks(b.n, 256), 256, 0, s>>>(b);
 }
 
 void launch_integrate(const Bufs &b, const SphConst &c, int src, int start_mode, int block,
@@ -970,7 +1220,10 @@ void launch_clock(const Bufs &b, const wsph_params &P, hipStream_t s) {
   k_clock<<<1, 1, 0, s>>>(b.st, P.dt_mode, P.cfl * P.spacing, P.inv_small, P.dt_max);
 }
 
-void launch_reset_state(const Bufs &b, hipStream_t s) { k_reset_state<<<1, 128, 0, s>>>(b.st); }
+void launch_reset_state(const Bufs &b, hipStream_t s) {
+  k_reset_state<<<1, 128, 0, s>>>(b.st);
+  if (b.scenes) k_batch_reset<<<64, 256, 0, s>>>(b, kMaxIters);
+}
 
 int persistent_max_blocks(const Bufs &b, const SphConst &c, int block, int *blocks_per_cu) {
   int dev = 0, ncu = 0, per = 0;
