@@ -36,6 +36,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -65,6 +67,8 @@ def parse():
                     help="persistent kernel blocks per CU (0: solver default)")
     ap.add_argument("--trace-steps", type=int, default=0,
                     help="print wall time and solver statistics of this many single steps first")
+    ap.add_argument("--autotune", choices=["off", "cached", "force"], default="cached",
+                    help="batch workload: launch shapes from the autotuner (cache or fresh sweep)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -170,6 +174,87 @@ def timed_run(cfg, net, args, trace: bool) -> dict:
                 value=s.n * (steps - nonconv) / elapsed)
 
 
+def bench_batch(args) -> None:
+    """The batch workload: 256 independent dam-break scenes in one solver,
+    launch shapes from the autotuner."""
+    import torch
+    from warpsph_amd import autotune, config, scene
+    from warpsph_amd.batch import BatchIISPH
+    start = {"zero": config.WARM_ZERO, "previous": config.WARM_PREVIOUS}.get(args.warm_start)
+    if start is None:
+        raise SystemExit("the batch workload runs the plain starts (--warm-start previous|zero)")
+    cfgs = [c.replace(warm_start=start) for c in scene.batch_configs(config.preset(args.preset))]
+    scs = [scene.build(c) for c in cfgs]
+
+    def make(shape_cfg):
+        return BatchIISPH([c.replace(**{k: getattr(shape_cfg, k) for k in autotune.SHAPE_KEYS})
+                           for c in cfgs], scs)
+
+    tuned = None
+    shape_cfg = cfgs[0]
+    if args.autotune != "off":
+        tuned = autotune.tune(make, cfgs[0], key=f"{args.preset}|start{start}|n{sum(x.n for x in scs)}",
+                              force=args.autotune == "force",
+                              log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+        shape_cfg = autotune.apply(cfgs[0], tuned["shape"])
+    b = make(shape_cfg)
+    lib, h = b._lib, b._h
+    from warpsph_amd import _native as N
+    for _ in range(args.warmup):
+        N.check("wsph_step", lib.wsph_step(h, None))
+    N.check("wsph_sync", lib.wsph_sync(h))
+    c0 = b.counters()
+    N.check("wsph_set_timing", lib.wsph_set_timing(h, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        N.check("wsph_step", lib.wsph_step(h, None))
+    N.check("wsph_sync", lib.wsph_sync(h))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    c1 = b.counters()
+    tm = (C.c_double * 8)()
+    N.check("wsph_last_timings", lib.wsph_last_timings(h, tm, 8))
+    if c1["status"]:
+        raise RuntimeError(f"device status {c1['status']} during the timed run")
+    steps = c1["steps"] - c0["steps"]
+    nonconv = c1["nonconverged"] - c0["nonconverged"]
+    ap_evals = c1["ap_evals"] - c0["ap_evals"]
+    last = b.scene_stats()
+    n = b.n
+    jac_ms = tm[2] / max(1, int(tm[6]))
+    jac_bytes = n * BYTES_PER_AP_EVAL * ap_evals / max(1, steps)
+    achieved = jac_bytes / (jac_ms * 1e-3) / 1e9
+    line = {
+        "metric": "fluid particle-steps/s, batch of independent scenes, rho-err<=0.1% per scene",
+        "value": n * (steps - nonconv) / elapsed, "unit": "fluid particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.preset, "scenes": b.nscenes, "fluid_particles": n,
+                   "boundary_particles": b.boundary_n, "spacing": cfgs[0].spacing,
+                   "dt": cfgs[0].dt0, "warm_start": args.warm_start,
+                   "shape": {k: getattr(shape_cfg, k) for k in autotune.SHAPE_KEYS}},
+        "jacobi_iters_per_step": (c1["iterations"] - c0["iterations"]) / max(1, steps),
+        "nonconverged_steps": nonconv,
+        "last_step_scene_iterations": {"min": int(last["iterations"].min()),
+                                       "median": float(np.median(last["iterations"])),
+                                       "max": int(last["iterations"].max())},
+        "phase_ms_per_step": {"neighbor_search": tm[0] / max(1, tm[6]),
+                              "predict_advection": tm[1] / max(1, tm[6]),
+                              "pressure_solve": jac_ms, "integrate": tm[3] / max(1, tm[6])},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "jacobi (k_j1_dev + k_j2_dev per A p evaluation)",
+                     "algorithmic_bytes_per_launch": jac_bytes, "avg_launch_ms": jac_ms},
+    }
+    if tuned is not None:
+        line["autotune"] = {"cached": tuned["cached"], "ms_per_step": tuned["ms_per_step"],
+                            "candidates": len(tuned.get("table", []))}
+    b.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy_mib:
@@ -181,6 +266,11 @@ def main():
     from warpsph_amd import config
     WARM.update(zero=config.WARM_ZERO, previous=config.WARM_PREVIOUS,
                 learned=config.WARM_LEARNED)
+    if config.preset(args.preset).scene == "batch":
+        if args.warm_start == "learned":
+            args.warm_start = "previous"
+        bench_batch(args)
+        return
 
     base = config.preset(args.preset)
     base = base.replace(jacobi_mode={"persistent": config.JACOBI_PERSISTENT,

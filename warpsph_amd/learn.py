@@ -18,6 +18,7 @@ from typing import Iterable, Optional
 
 import numpy as np
 
+from . import _native as N
 from . import config as cfgmod
 from . import scene as scn
 from .mlp import N_IN, PressureNet
@@ -57,6 +58,48 @@ def generate_pairs(cfgs: Iterable[cfgmod.SolverConfig], steps: int, per_step: in
             log(f"scene {k}: n={s.n} steps={steps} iters/step={c['iterations'] / max(c['steps'], 1):.2f} "
                 f"{time.time() - t0:.1f}s")
         s.close()
+    return torch.cat(feats), torch.cat(targs)
+
+
+def generate_pairs_batched(cfgs, steps: int, per_step: int, seed: int = 0,
+                           device: str = "cuda:0", log=None, shape: Optional[dict] = None):
+    """As ``generate_pairs``, with all scenes in one batched solver
+    (``BatchIISPH``): per step, ``per_step`` random rows of every scene."""
+    from .batch import BatchIISPH
+    torch = _torch()
+    cfgs = [c.replace(warm_start=cfgmod.WARM_PREVIOUS, **(shape or {})) for c in cfgs]
+    b = BatchIISPH(cfgs, device=device)
+    N.check("wsph_set_pair_logging", b._lib.wsph_set_pair_logging(b._h, 1))
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    counts = torch.as_tensor(b.counts, dtype=torch.float64, device=device)
+    offs = torch.as_tensor(b.offsets[:-1], dtype=torch.int64, device=device)
+    take = torch.minimum(torch.full_like(counts, per_step), counts).to(torch.int64)
+    f = torch.empty((b.n, 8), dtype=torch.float32, device=device)
+    t = torch.empty(b.n, dtype=torch.float32, device=device)
+    feats, targs = [], []
+    t0 = time.time()
+    for k in range(steps):
+        b.step()
+        torch.cuda.synchronize()
+        N.check("wsph_get_pairs", b._lib.wsph_get_pairs(b._h, f.data_ptr(), t.data_ptr()))
+        u = torch.rand((b.nscenes, per_step), device=device, generator=gen, dtype=torch.float64)
+        idx = (offs[:, None] + (u * counts[:, None]).to(torch.int64))
+        # scenes smaller than per_step contribute each row once
+        keep = torch.arange(per_step, device=device)[None, :] < take[:, None]
+        small = counts < per_step
+        if bool(small.any()):
+            idx = torch.where(small[:, None], offs[:, None] + torch.arange(per_step, device=device)[None, :],
+                              idx)
+        idx = idx[keep]
+        feats.append(f[idx])
+        targs.append(t[idx])
+        if log and (k + 1) % 50 == 0:
+            c = b.counters()
+            log(f"batch pairs: step {k + 1}/{steps}, iters/step {c['iterations'] / max(c['steps'], 1):.2f}, "
+                f"{time.time() - t0:.1f}s")
+    torch.cuda.synchronize()
+    b.close()
     return torch.cat(feats), torch.cat(targs)
 
 

@@ -2,8 +2,9 @@
 
     python -m warpsph_amd.train_predictor --out warpsph_amd/pressure_net.npy
 
-1. (features, target) pairs from the batch workload's scenes (previous-step
-   start, pair logging on; ``--scenes`` of them, ``--steps`` steps each,
+1. (features, target) pairs from the batch workload's scenes, solved together
+   in one batched solver (previous-step start, pair logging on; ``--scenes``
+   of them, ``--steps`` steps each,
    ``--per-step`` random rows kept per step; weakly coupled particles,
    |a_ii / a_ref| < 0.05, are dropped since the solver never asks the network
    about them);
@@ -40,6 +41,8 @@ def main(argv=None) -> int:
     ap.add_argument("--eval-steps", type=int, default=500)
     ap.add_argument("--out", default=None, help="save the trained parameters (.npy)")
     ap.add_argument("--init", default=None, help="start from these parameters (.npy)")
+    ap.add_argument("--sequential", action="store_true",
+                    help="generate pairs scene by scene instead of in one batched solver")
     a = ap.parse_args(argv)
     import torch
 
@@ -48,8 +51,11 @@ def main(argv=None) -> int:
 
     t0 = time.time()
     cfgs = scn.batch_configs(cfgmod.batch_256())[:a.scenes]
-    X, T = learn.generate_pairs(cfgs, a.steps, a.per_step, seed=a.seed,
-                                log=log if a.scenes <= 16 else None)
+    if a.sequential:
+        X, T = learn.generate_pairs(cfgs, a.steps, a.per_step, seed=a.seed,
+                                    log=log if a.scenes <= 16 else None)
+    else:
+        X, T = learn.generate_pairs_batched(cfgs, a.steps, a.per_step, seed=a.seed, log=log)
     keep = X[:, 1].abs() >= 0.05
     X, T = X[keep].contiguous(), T[keep].contiguous()
     t_gen = time.time() - t0
