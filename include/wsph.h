@@ -118,6 +118,9 @@ typedef struct wsph_params {
   int block;        /* threads per block of the neighbour passes (64..1024) */
   int coop_blocks_per_cu; /* persistent kernel blocks per CU (0 = default 2) */
   int chunk_iters;  /* WSPH_JACOBI_DEVICE: iterations queued per chunk (0 = 2) */
+  int graph_iters;  /* WSPH_JACOBI_DEVICE, > 0: wsph_step replays a captured hipGraph of
+                       the step's grid build, advection passes, start value and first
+                       graph_iters Jacobi iterations; later iterations run as chunks */
 } wsph_params;
 
 /* Per-step statistics (what IISPH.step computes and prints, IISPH.py:1200-1371) */

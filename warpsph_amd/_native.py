@@ -36,7 +36,7 @@ class WsphParams(C.Structure):
         ("grid_dims", C.c_int * 3), ("bgrid_dims", C.c_int * 3),
         ("cell_layout", C.c_int), ("max_ff", C.c_int), ("max_fs", C.c_int),
         ("jacobi_mode", C.c_int), ("block", C.c_int),
-        ("coop_blocks_per_cu", C.c_int), ("chunk_iters", C.c_int),
+        ("coop_blocks_per_cu", C.c_int), ("chunk_iters", C.c_int), ("graph_iters", C.c_int),
     ]
 
 
@@ -81,6 +81,7 @@ def make_params(cfg: SolverConfig, grid_dims, bgrid_dims) -> WsphParams:
     p.block = cfg.block
     p.coop_blocks_per_cu = cfg.coop_blocks_per_cu
     p.chunk_iters = cfg.chunk_iters
+    p.graph_iters = cfg.graph_iters
     return p
 
 

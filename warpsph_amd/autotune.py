@@ -2,8 +2,9 @@
 
 Times a few steps of the actual workload for each candidate shape: threads per
 block of the neighbour passes, Jacobi driver (device-driven chunks of 1, 2 or
-4 iterations, or the persistent cooperative kernel at 1 or 2 blocks per CU)
-and cell layout (Morton or linear). Every candidate computes the same numbers
+4 iterations, the same with the step's front half replayed as a captured
+hipGraph holding 2 or 4 iterations, or the persistent cooperative kernel at 1
+or 2 blocks per CU) and cell layout (Morton or linear). Every candidate computes the same numbers
 (the neighbour order and the float operations do not depend on the shape),
 so the candidates run identical work from the same initial state and only
 their wall time differs. Results are cached per device and workload in a
@@ -22,15 +23,18 @@ from . import config as cfgmod
 DEFAULT_CACHE = Path(os.environ.get("WSPH_AUTOTUNE_CACHE",
                                     Path.home() / ".cache" / "warpsph_amd" / "autotune.json"))
 
-SHAPE_KEYS = ("block", "jacobi_mode", "chunk_iters", "coop_blocks_per_cu", "cell_layout")
+SHAPE_KEYS = ("block", "jacobi_mode", "chunk_iters", "coop_blocks_per_cu", "cell_layout",
+              "graph_iters")
 
 
 def candidates(blocks: Iterable[int] = (128, 256, 512),
                layouts: Iterable[int] = (cfgmod.LAYOUT_MORTON, cfgmod.LAYOUT_LINEAR)) -> List[dict]:
-    drivers = [dict(jacobi_mode=cfgmod.JACOBI_DEVICE, chunk_iters=c, coop_blocks_per_cu=0)
-               for c in (1, 2, 4)]
-    drivers += [dict(jacobi_mode=cfgmod.JACOBI_PERSISTENT, chunk_iters=0, coop_blocks_per_cu=c)
-                for c in (1, 2)]
+    drivers = [dict(jacobi_mode=cfgmod.JACOBI_DEVICE, chunk_iters=c, coop_blocks_per_cu=0,
+                    graph_iters=0) for c in (1, 2, 4)]
+    drivers += [dict(jacobi_mode=cfgmod.JACOBI_DEVICE, chunk_iters=2, coop_blocks_per_cu=0,
+                     graph_iters=g) for g in (2, 4)]
+    drivers += [dict(jacobi_mode=cfgmod.JACOBI_PERSISTENT, chunk_iters=0, coop_blocks_per_cu=c,
+                     graph_iters=0) for c in (1, 2)]
     return [dict(block=b, cell_layout=l, **d) for b in blocks for l in layouts for d in drivers]
 
 

@@ -71,6 +71,7 @@ class SolverConfig:
     block: int = 256
     coop_blocks_per_cu: int = 0
     chunk_iters: int = 0
+    graph_iters: int = 0           # >0: replay the step as a captured hipGraph
     extra: dict = field(default_factory=dict)
 
     # ---- derived quantities (wp_sph/main.py:57-60) ---------------------

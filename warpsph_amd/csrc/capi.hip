@@ -176,6 +176,10 @@ struct wsph_solver {
   int *done_flags = nullptr;
   hipEvent_t chunk_ev[2] = {};
   int step_count = 0;
+  // captured step graphs, one per fluid buffer parity at step start
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  int graph_src = 0;  // Jacobi source buffer after the start value (graph mode)
   // batched scenes (nscenes == 0: one scene described by the params)
   int nscenes = 0;
   std::vector<int> scene_nf, scene_nb;
@@ -412,7 +416,17 @@ int wsph_create(const wsph_params *params, wsph_solver **out) {
   return WSPH_OK;
 }
 
+static void drop_graphs(wsph_solver *s) {
+  for (int k = 0; k < 2; ++k) {
+    if (s->gexec[k]) (void)hipGraphExecDestroy(s->gexec[k]);
+    if (s->graph[k]) (void)hipGraphDestroy(s->graph[k]);
+    s->gexec[k] = nullptr;
+    s->graph[k] = nullptr;
+  }
+}
+
 static void free_fluid(wsph_solver *s) {
+  drop_graphs(s);
   for (int k = 0; k < 2; ++k) {
     dfree(s->pos[k]);
     dfree(s->vel[k]);
@@ -503,6 +517,7 @@ int wsph_set_tables(wsph_solver *s, const float *w_table, int w_size, const floa
                     int grad_size) {
   if (!s || !w_table || !grad_table || w_size < 2 || grad_size < 2)
     return fail(WSPH_E_ARG, "tables need at least 2 samples");
+  drop_graphs(s);
   dfree(s->wt);
   dfree(s->gt);
   HIP_TRY(dalloc(&s->wt, w_size));
@@ -550,6 +565,7 @@ static int sort_points(wsph_solver *s, GridBufs &g, const float4 *pos, const int
 int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) {
   if (!s || nb < 0 || (nb > 0 && !xb)) return fail(WSPH_E_ARG, "bad boundary arguments");
   if (!s->wt) return fail(WSPH_E_ARG, "call wsph_set_tables first");
+  drop_graphs(s);
   dfree(s->bpos);
   dfree(s->bvel);
   dfree(s->borig);
@@ -752,6 +768,32 @@ int wsph_predict_advection(wsph_solver *s) {
   return WSPH_OK;
 }
 
+// Device-driven Jacobi loop: iterations queued in chunks, one chunk ahead of
+// the host's look at the device's stop flag. `queued` iterations (with their
+// done flags in done_flags[0 .. m0)) were enqueued already.
+static int device_loop(wsph_solver *s, const Bufs &b, int src, int queued, int m0) {
+  const wsph_params &P = s->P;
+  const int C = P.chunk_iters > 0 ? P.chunk_iters : 2;
+  const int max_it = P.force_iters > 0 ? P.force_iters : P.max_iters;
+  const int tag = s->step_count;
+  for (int m = m0; queued < max_it; ++m) {
+    const int cnt = C < max_it - queued ? C : max_it - queued;
+    launch_jacobi_chunk(b, s->C, P, src, cnt, P.block, s->stream);
+    queued += cnt;
+    HIP_TRY(hipMemcpyAsync(&s->done_flags[m], &s->st->done_tag, sizeof(int),
+                           hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipEventRecord(s->chunk_ev[m & 1], s->stream));
+    if (queued >= max_it) break;
+    if (m >= 1) {
+      HIP_TRY(hipEventSynchronize(s->chunk_ev[(m - 1) & 1]));
+      if (__atomic_load_n(&s->done_flags[m - 1], __ATOMIC_ACQUIRE) == tag) break;
+    }
+  }
+  launch_batch_finalize(b, s->stream);
+  s->final_dp = -1;
+  return WSPH_OK;
+}
+
 int wsph_pressure_solve(wsph_solver *s, wsph_step_stats *stats) {
   if (!s || !s->advected) return fail(WSPH_E_ARG, "run wsph_predict_advection first");
   rec(s, 2);
@@ -786,27 +828,8 @@ int wsph_pressure_solve(wsph_solver *s, wsph_step_stats *stats) {
     }
     s->final_dp = 0;
   } else if (P.jacobi_mode == WSPH_JACOBI_DEVICE) {
-    // iterations queued in chunks, one chunk ahead of the host's look at the
-    // device's stop flag
-    const int C = P.chunk_iters > 0 ? P.chunk_iters : 2;
-    const int max_it = P.force_iters > 0 ? P.force_iters : P.max_iters;
-    const int tag = s->step_count;
-    int queued = 0;
-    for (int m = 0;; ++m) {
-      const int cnt = C < max_it - queued ? C : max_it - queued;
-      launch_jacobi_chunk(b, s->C, P, src, cnt, P.block, s->stream);
-      queued += cnt;
-      HIP_TRY(hipMemcpyAsync(&s->done_flags[m], &s->st->done_tag, sizeof(int),
-                             hipMemcpyDeviceToHost, s->stream));
-      HIP_TRY(hipEventRecord(s->chunk_ev[m & 1], s->stream));
-      if (queued >= max_it) break;
-      if (m >= 1) {
-        HIP_TRY(hipEventSynchronize(s->chunk_ev[(m - 1) & 1]));
-        if (__atomic_load_n(&s->done_flags[m - 1], __ATOMIC_ACQUIRE) == tag) break;
-      }
-    }
-    launch_batch_finalize(b, s->stream);
-    s->final_dp = -1;
+    int rc = device_loop(s, b, src, 0, 0);
+    if (rc) return rc;
   } else {
     if (s->nscenes > 0)
       return fail(WSPH_E_UNSUPPORTED, "batched scenes need the device or persistent Jacobi driver");
@@ -886,8 +909,113 @@ int wsph_integrate(wsph_solver *s, wsph_step_stats *stats) {
   return WSPH_OK;
 }
 
+// Enqueue the step's front half on the stream (being captured): state reset,
+// grid build with buffer set c as the input, neighbour lists, advection
+// passes, start value, A p0 and the first graph_iters Jacobi iterations.
+static int enqueue_front(wsph_solver *s, int c) {
+  const int o = c ^ 1;
+  const wsph_params &P = s->P;
+  s->cur = c;
+  launch_reset_state(make_bufs(s), s->stream);
+  int rc = sort_points(s, s->fg, s->pos[c], s->orig[c], s->n, s->slot, s->rank, s->tmp, s->perm,
+                       s->sid[c], 0);
+  if (rc) return rc;
+  gather_fluid(s->perm, s->n, s->pos[c], s->vel[c], s->pprev[c], s->orig[c], s->pos[o], s->vel[o],
+               s->pprev[o], s->orig[o], s->stream);
+  if (s->nscenes > 0) gather_i32(s->perm, s->n, s->sid[c], s->sid[o], s->stream);
+  s->cur = o;
+  if (P.warm_start == WSPH_WARM_EXTERNAL && s->n > 0)
+    k_gather_by_orig<<<nblk(s->n, 256), 256, 0, s->stream>>>(s->p_ext, s->orig[o], s->n,
+                                                              s->pstart);
+  const Bufs b = make_bufs(s);
+  launch_lists(b, s->C, P.block, s->stream);
+  const bool learned = P.warm_start == WSPH_WARM_LEARNED;
+  const bool plain = P.vis_mu == 0.0 && P.xsph == 0.0;
+  launch_density(b, s->C, plain, P.block, s->stream);
+  if (!plain) launch_vadv(b, s->C, P.block, s->stream);
+  launch_assemble(b, s->C, P.warm_start, P.block, s->stream);
+  if (learned || s->log_pairs) {
+    Weights W;
+    if (learned) W = weights_of(s->net->dev);
+    launch_learned_start(b, s->C, learned ? &W : nullptr, s->stream);
+  }
+  int src = 0;
+  if (P.warm_start != WSPH_WARM_ZERO) {
+    launch_j1(b, s->C, 0, P.block, s->stream);
+    launch_j2(b, s->C, 0, -1, P.block, s->stream);
+    k_add_ap_evals<<<1, 1, 0, s->stream>>>(s->st, 1);
+    src = 1;
+  }
+  s->graph_src = src;
+  const int max_it = P.force_iters > 0 ? P.force_iters : P.max_iters;
+  const int K = P.graph_iters < max_it ? P.graph_iters : max_it;
+  launch_jacobi_chunk(b, s->C, P, src, K, P.block, s->stream);
+  HIP_TRY(hipMemcpyAsync(&s->done_flags[0], &s->st->done_tag, sizeof(int), hipMemcpyDeviceToHost,
+                         s->stream));
+  HIP_TRY(hipGetLastError());
+  return WSPH_OK;
+}
+
+static int capture_front(wsph_solver *s, int c) {
+  const int saved = s->cur;
+  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_front(s, c);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s->stream, &g);
+  s->cur = saved;
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(WSPH_E_HIP, "step capture: %s", hipGetErrorString(e));
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(WSPH_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  }
+  s->graph[c] = g;
+  s->gexec[c] = x;
+  return WSPH_OK;
+}
+
+// wsph_step with the front half replayed from a captured graph.
+static int graph_step(wsph_solver *s, wsph_step_stats *stats) {
+  const wsph_params &P = s->P;
+  const bool learned = P.warm_start == WSPH_WARM_LEARNED;
+  if (learned && !s->net) return fail(WSPH_E_ARG, "learned start needs wsph_set_predictor");
+  if (learned || s->log_pairs) {
+    int rc = ensure_learn_bufs(s);  // no allocation may happen inside the capture
+    if (rc) return rc;
+  }
+  const int c = s->cur;
+  if (!s->gexec[c]) {
+    int rc = capture_front(s, c);
+    if (rc) return rc;
+  }
+  if (s->timing && s->t_recorded - s->t_processed >= wsph_solver::kRing) timing_fold_one(s);
+  s->step_count++;
+  rec(s, 0);
+  HIP_TRY(hipGraphLaunch(s->gexec[c], s->stream));
+  HIP_TRY(hipEventRecord(s->chunk_ev[0], s->stream));
+  s->cur = c ^ 1;
+  s->searched = s->advected = true;
+  rec(s, 1);
+  rec(s, 2);
+  const int max_it = P.force_iters > 0 ? P.force_iters : P.max_iters;
+  const int K = P.graph_iters < max_it ? P.graph_iters : max_it;
+  int rc = device_loop(s, make_bufs(s), s->graph_src, K, 1);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  s->solved = true;
+  return wsph_integrate(s, stats);
+}
+
 int wsph_step(wsph_solver *s, wsph_step_stats *stats) {
   int rc;
+  if (!s) return fail(WSPH_E_ARG, "null solver");
+  if (s->P.graph_iters > 0 && s->P.jacobi_mode == WSPH_JACOBI_DEVICE && s->n > 0)
+    return graph_step(s, stats);
   if ((rc = wsph_neighbor_search(s))) return rc;
   if ((rc = wsph_predict_advection(s))) return rc;
   if ((rc = wsph_pressure_solve(s, stats))) return rc;
@@ -1144,12 +1272,14 @@ int wsph_get_scene_stats(wsph_solver *s, int *iterations, int *converged, double
 
 int wsph_set_predictor(wsph_solver *s, wsph_mlp *m) {
   if (!s) return fail(WSPH_E_ARG, "null solver");
+  if (m != s->net) drop_graphs(s);
   s->net = m;
   return WSPH_OK;
 }
 
 int wsph_set_pair_logging(wsph_solver *s, int on) {
   if (!s) return fail(WSPH_E_ARG, "null solver");
+  if ((on != 0) != s->log_pairs) drop_graphs(s);
   s->log_pairs = on != 0;
   s->have_pairs = false;
   return WSPH_OK;
