@@ -297,7 +297,9 @@ def main():
     achieved = jac_bytes / (jac_ms * 1e-3) / 1e9
     workload = args.preset
     line = {
-        "metric": "fluid particle-steps/s @1M, rho-err<=0.1%; Jacobi iters/step learned vs plain",
+        "metric": ("fluid particle-steps/s @1M, rho-err<=0.1%; Jacobi iters/step learned vs plain"
+                   if args.preset == "dambreak_1m" else
+                   f"fluid particle-steps/s, {args.preset}, rho-err<=0.1%"),
         "value": r["value"],
         "unit": "fluid particle-steps/s",
         "n_gpus": 1,

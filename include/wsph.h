@@ -239,6 +239,21 @@ int wsph_get_counters(wsph_solver *s, long long *out, int nout);
  * kept on the device so that long runs need no per-step synchronisation. */
 int wsph_get_history(wsph_solver *s, wsph_step_record *out, int max_records, int *count);
 
+/* ---- scene generation (host only, no GPU work) --------------------------
+ * Poisson-disk sample of a boundary surface with minimum distance `radius`
+ * (Bridson's algorithm run on the surface: candidates at distance
+ * [radius, 2 radius) along the tangent plane are projected onto the surface;
+ * a splitmix64 stream seeded with `seed` drives it, so the points depend only
+ * on the arguments). WSPH_SURF_PLANE: params = {x0, z0, x1, z1, y}, the
+ * rectangle [x0, x1] x [z0, z1] at height y; WSPH_SURF_TORUS: params =
+ * {cx, cy, cz, R, r}, axis along y. Writes *count and, when out is not NULL,
+ * the points as float[count][3] (host; WSPH_E_CAPACITY if count > max_points).
+ * Replaces the reference's mesh-sampled boundaries (model.py:14-60) for the
+ * torus-and-plane scene. */
+enum { WSPH_SURF_PLANE = 0, WSPH_SURF_TORUS = 1 };
+int wsph_poisson_surface(int shape, const double *params, double radius, uint64_t seed,
+                         int max_points, float *out, int *count);
+
 /* ---- batched scenes (no reference counterpart) -------------------------
  * Many independent scenes solved together: call after wsph_create and before
  * the particle calls (wsph_set_boundary / wsph_set_fluid), whose

@@ -9,7 +9,8 @@ oracle/learn_oracle.py and the IISPH oracle.
 * a constant network gives the start max(0, carry + P_ref * b3): with that
   start the oracle reproduces the GPU trajectory bit for bit, and the
   recorded a_ii-weighted start errors equal the CPU's within 1e-5 relative;
-* a network that predicts badly is switched off by the judge after one step.
+* a network that predicts badly is switched off by the judge, and is then
+  evaluated and judged every 8th step only.
 """
 import sys
 
@@ -39,6 +40,20 @@ def _const_net(b3):
     p = np.zeros(MO.N_PARAM, np.float32)
     p[MO.OFF_B3] = b3
     return PressureNet(max_batch=1024, params=p)
+
+
+RETRY_EVERY = 8  # a switched-off network is evaluated every 8th step
+
+
+def _evaluated(r):
+    return r["start_source"] == config.WARM_LEARNED or r["step"] % RETRY_EVERY == 0
+
+
+def _next_source(r):
+    """The judge: the next step starts from the network only if it was
+    evaluated and its start was no worse than the carried pressure."""
+    ok = _evaluated(r) and r["start_err_learned"] <= r["start_err_carry"]
+    return config.WARM_LEARNED if ok else config.WARM_PREVIOUS
 
 
 def _cfg():
@@ -118,16 +133,14 @@ def test_constant_network_start_matches_oracle(orc):
         info = o.step(p0=p0)
         assert np.all(np.abs(info["a_ii"] / LO.a_ref(cfg, cfg.dt0)) >= LO.WEAK_COUPLING)
         pf = info["p"]
-        # the GPU weights with float32 a_ii * k1 / dt^2: compare within 1e-5
-        en = LO.judge_error(net_start, pf, info["a_ii"], cfg, cfg.dt0)
-        ec = LO.judge_error(carry, pf, info["a_ii"], cfg, cfg.dt0)
-        assert abs(hist[k]["start_err_learned"] - en) <= 1e-5 * max(en, 1.0)
-        assert abs(hist[k]["start_err_carry"] - ec) <= 1e-5 * max(ec, 1.0)
-        # the judge: the next step uses the network only if it was no worse
+        if _evaluated(hist[k]):
+            # the GPU weights with float32 a_ii * k1 / dt^2: compare within 1e-5
+            en = LO.judge_error(net_start, pf, info["a_ii"], cfg, cfg.dt0)
+            ec = LO.judge_error(carry, pf, info["a_ii"], cfg, cfg.dt0)
+            assert abs(hist[k]["start_err_learned"] - en) <= 1e-5 * max(en, 1.0)
+            assert abs(hist[k]["start_err_carry"] - ec) <= 1e-5 * max(ec, 1.0)
         if k + 1 < steps:
-            a, b = hist[k]["start_err_learned"], hist[k]["start_err_carry"]
-            want = config.WARM_LEARNED if a <= b else config.WARM_PREVIOUS
-            assert hist[k + 1]["start_source"] == want
+            assert hist[k + 1]["start_source"] == _next_source(hist[k])
         assert hist[k]["iterations"] == info["iterations"]
         carry = info["p_carry"]
     assert hist[0]["start_source"] == config.WARM_LEARNED
@@ -140,17 +153,17 @@ def test_judge_switches_off_a_bad_network():
     _announce("learned: judge, bad network")
     net = _const_net(-30.0)  # start max(0, carry - 30 P_ref) = 0 everywhere
     s = _solver(cfg, net)
-    for _ in range(12):
+    for _ in range(20):
         s.step()
     h = s.history()
     # step 0: carry and network both start from 0 (a tie keeps the network)
     assert h[0]["start_err_learned"] == h[0]["start_err_carry"]
     assert h[0]["start_source"] == config.WARM_LEARNED
-    # the rule: the next step uses the network only if its start was no worse
     for a, b in zip(h, h[1:]):
-        want = (config.WARM_LEARNED if a["start_err_learned"] <= a["start_err_carry"]
-                else config.WARM_PREVIOUS)
-        assert b["start_source"] == want
+        assert b["start_source"] == _next_source(a)
+        # not evaluated: no errors recorded
+        if not _evaluated(a):
+            assert a["start_err_learned"] == 0 and a["start_err_carry"] == 0
     # the network's start (0) loses to the carried pressure at some step
     assert any(r["start_source"] == config.WARM_PREVIOUS for r in h)
 

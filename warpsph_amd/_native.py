@@ -129,6 +129,7 @@ SIGNATURES = [
     ("wsph_mlp_get_params", _I, [_P, _P]),
     ("wsph_mlp_forward", _I, [_P, _P, _I, _P]),
     ("wsph_mlp_train_step", _I, [_P, _P, _P, _I, C.c_float, C.POINTER(C.c_float), _P]),
+    ("wsph_poisson_surface", _I, [_I, _P, C.c_double, C.c_uint64, _I, _P, C.POINTER(C.c_int)]),
     ("wsph_set_scenes", _I, [_P, _I, _P, _P, _P, _P]),
     ("wsph_get_scene_stats", _I, [_P, _P, _P, _P]),
     ("wsph_set_predictor", _I, [_P, _P]),
@@ -158,6 +159,15 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
         raise RuntimeError(
             f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
             "or `python -m warpsph_amd.build`; the solver has no CPU fallback")
+    # torch and the library share the HIP runtime, and torch must bring it up
+    # first: with the library loaded before torch's first device query, the
+    # device was not found on the MI355X host. Querying is harmless without a
+    # GPU.
+    try:
+        import torch
+        torch.cuda.is_available()
+    except ImportError:
+        pass
     lib = C.CDLL(str(p))
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name)

@@ -135,13 +135,16 @@ def dambreak_1m() -> SolverConfig:
 
 
 def torus_4m() -> SolverConfig:
-    """~4M fluid particles at 0.005 m falling onto a torus (R = 0.3, r = 0.1)
-    and a ground plane, both Poisson-disk sampled at 0.005 m from seed 1;
-    300 steps."""
+    """160^3 = 4,096,000 fluid particles at 0.005 m, block corner at
+    (0.2, 0.25, 0.2), falling onto a torus (R = 0.3, r = 0.1, centre
+    (0.6, 0.105, 0.6), axis vertical) lying on a 2 x 2 m ground plane at
+    y = 0; both surfaces Poisson-disk sampled at 0.005 m from seed 1 (plane)
+    and 2 (torus); no walls; dt = 1e-3 fixed, XSPH 0.01; 300 steps."""
     return dambreak_8k().replace(
-        spacing=0.005, box=(1.2, 1.6, 1.2), boundary_layers=0,
-        scene="torus", fluid_cells=(160, 160, 160), seed=1, steps=300,
-        extra={"torus_R": 0.3, "torus_r": 0.1})
+        spacing=0.005, box=(2.0, 1.1, 2.0), boundary_layers=0,
+        scene="torus", fluid_cells=(160, 160, 160), fluid_origin=(0.2, 0.25, 0.2), seed=1,
+        steps=300, extra={"torus_R": 0.3, "torus_r": 0.1, "torus_center": (0.6, 0.105, 0.6),
+                          "plane": (-0.4, -0.4, 1.6, 1.6, 0.0)})
 
 
 def batch_256() -> SolverConfig:

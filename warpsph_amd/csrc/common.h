@@ -35,6 +35,9 @@ constexpr int kMaxLdsTable = 16384; // floats per table kept in LDS (64 KiB)
 // particle with |a_ii / a_ref| < kWeakCoupling keeps the carried pressure
 // (its pressure barely moves the density and its value is unreliable).
 constexpr float kFeatClamp = 64.0f;
+// While the network is switched off it is evaluated (and judged) every
+// kRetryEvery-th step only.
+constexpr int kRetryEvery = 8;
 constexpr float kWeakCoupling = 0.05f;
 __host__ __device__ inline float clamp_feat(float v) {
   return v > kFeatClamp ? kFeatClamp : (v < -kFeatClamp ? -kFeatClamp : v);
@@ -93,6 +96,7 @@ struct DevState {
   unsigned bar_seen;            // arrivals seen by a barrier wait that timed out
   int step_count;             // steps recorded in the history ring
   int net_off;                // learned start: 1 = the network lost the last comparison
+  int net_eval;               // learned start: the network was evaluated this step
   unsigned long long res_sum[kMaxIters];
   unsigned res_cnt[kMaxIters];
 };

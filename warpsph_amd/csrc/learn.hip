@@ -11,6 +11,8 @@
 //   2. optionally writes the 8-feature row (training pairs),
 //   3. stages the rows of its 64 particles in LDS and runs the network on
 //      them as two 32-row MFMA tiles (BF16 operands, FP32 accumulation),
+// A network switched off by the judge is evaluated every kRetryEvery-th step
+// only (DevState::net_eval records whether it ran).
 //   4. writes the start value p0 = max(0, carry + P_ref * y) (the carried
 //      value where |a_ii / a_ref| < kWeakCoupling, zero inside the a_ii guard
 //      band) to aux.z (always) and to the Jacobi iterate dp[0].w
@@ -34,8 +36,15 @@ __global__ __launch_bounds__(64 * kLearnWaves) void k_learned_start(Bufs b, SphC
   __shared__ float act[kInfer ? kLearnWaves : 1][32][65];
   __shared__ float ys[kLearnWaves][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if constexpr (kInfer) {
+    // a switched-off network is retried every kRetryEvery-th step (the same
+    // decision in every block); without logging nothing else is needed
+    const bool eval = !b.st->net_off || b.st->step_count % kRetryEvery == 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) b.st->net_eval = eval ? 1 : 0;
+    if (!eval && !b.feat) return;
+  }
   const int i = particle_index();
-  const bool live = i < b.n;  // no early exit: forward_tile has block barriers
+  const bool live = i < b.n;  // no early exit below: forward_tile has block barriers
   float f[WSPH_MLP_IN] = {};
   float carry = 0.0f, a_ii = 0.0f;
   if (live) {

@@ -811,7 +811,7 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
     if (b.target || b.judge) {
       const float carry = b.pprev[i];  // this step's carried start value
       if (b.target) b.target[i] = clamp_feat((di.w - carry) * c.inv_pref);
-      if (b.judge) {
+      if (b.judge && b.st->net_eval) {
         // start errors weighted by min(|a_ii / a_ref|, 1): a particle's share
         // of the density residual scales with a_ii
         float wgt = fabsf(a * c.feat_k1 / (dt * dt));
@@ -945,7 +945,7 @@ __global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode) {
     if (start_mode == WSPH_WARM_LEARNED) {
       // net_off still holds the choice this step's feature pass made
       if (st->net_off) r.start_source = WSPH_WARM_PREVIOUS;
-      if (b.judge && b.n > 0) {
+      if (b.judge && b.n > 0 && st->net_eval) {
         r.err_net = sqrt(en / b.n);
         r.err_carry = sqrt(ec / b.n);
         // the carried pressure judges the network for the next step

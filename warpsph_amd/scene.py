@@ -105,6 +105,44 @@ def dam_break(cfg: SolverConfig) -> Scene:
     return Scene(x32, xb32, np.zeros_like(xb32), xb32.shape[0], lo, hi)
 
 
+def poisson_surface(shape: str, params: Sequence[float], radius: float, seed: int) -> np.ndarray:
+    """Poisson-disk samples (float32 [n, 3]) of a plane rectangle
+    (x0, z0, x1, z1, y) or a torus (cx, cy, cz, R, r), axis along y, with
+    minimum distance ``radius`` (wsph_poisson_surface; host code)."""
+    import ctypes as C
+    from . import _native as N
+    lib = N.load()
+    kind = {"plane": 0, "torus": 1}[shape]
+    prm = (C.c_double * 5)(*[float(v) for v in params])
+    cnt = C.c_int()
+    N.check("wsph_poisson_surface",
+            lib.wsph_poisson_surface(kind, prm, float(radius), int(seed), 0, None, C.byref(cnt)))
+    out = np.zeros((cnt.value, 3), dtype=np.float32)
+    N.check("wsph_poisson_surface",
+            lib.wsph_poisson_surface(kind, prm, float(radius), int(seed), cnt.value,
+                                     out.ctypes.data, C.byref(cnt)))
+    return out
+
+
+def torus_scene(cfg: SolverConfig) -> Scene:
+    """A fluid lattice block above a torus lying on a ground plane; both
+    surfaces Poisson-disk sampled at the fluid spacing (plane from ``seed``,
+    torus from ``seed + 1``). No walls: fluid that leaves the plane falls
+    freely. extra: torus_R, torus_r, torus_center, plane (x0, z0, x1, z1, y)."""
+    ex = cfg.extra
+    d = cfg.spacing
+    R, r = ex.get("torus_R", 0.3), ex.get("torus_r", 0.1)
+    cx, cy, cz = ex.get("torus_center", (0.6, 0.1 + d, 0.6))
+    plane = ex.get("plane", (-0.4, -0.4, 1.6, 1.6, 0.0))
+    xp = poisson_surface("plane", plane, d, cfg.seed)
+    xt = poisson_surface("torus", (cx, cy, cz, R, r), d, cfg.seed + 1)
+    x = lattice_block(cfg.fluid_cells, d, cfg.fluid_origin, cfg.jitter, cfg.seed)
+    x32 = _c32(x)
+    xb32 = _c32(np.concatenate([xp, xt]))
+    lo, hi = _bounds(x32, xb32)
+    return Scene(x32, xb32, np.zeros_like(xb32), xp.shape[0], lo, hi)
+
+
 def batch_configs(cfg: SolverConfig) -> List[SolverConfig]:
     """Configurations of the independent dam-break scenes of a batch.
 
@@ -145,13 +183,11 @@ def batch_configs(cfg: SolverConfig) -> List[SolverConfig]:
 GENERATORS = {
     "reference_box": reference_box,
     "dam_break": dam_break,
+    "torus": torus_scene,
 }
 
 
 def build(cfg: SolverConfig) -> Scene:
-    if cfg.scene == "torus":
-        from .torus import torus_scene
-        return torus_scene(cfg)
     try:
         gen = GENERATORS[cfg.scene]
     except KeyError:
