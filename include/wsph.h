@@ -223,7 +223,10 @@ int wsph_hash_keys(const float *x, int n, float cell_width, const int dims[3],
 /* Phase timing with HIP events on the solver's stream, summed over every
  * step since wsph_set_timing(s, 1) (which resets the sums): out[0..3] ms of
  * neighbour search, advection, pressure solve and integration, out[4] ms of
- * whole steps, out[5] persistent-kernel grid size, out[6] steps recorded. */
+ * whole steps, out[5] persistent-kernel grid size, out[6] steps recorded,
+ * out[7] and out[8] ms of the J1 and J2 launches (A p, IISPH.py:431-518) of
+ * each step's first Jacobi iteration (device-driven loop without a captured
+ * graph), out[9] steps in those two sums. */
 int wsph_last_timings(wsph_solver *s, double *out, int nout);
 int wsph_set_timing(wsph_solver *s, int enable);
 

@@ -167,8 +167,13 @@ struct wsph_solver {
   // timing: a ring of per-step event sets, read back lazily so that timed
   // steps need no host synchronisation
   bool timing = false;
-  static constexpr int kRing = 64, kEv = 5;
+  // per step: 5 phase boundaries, then 3 events around J1 and J2 of the
+  // first Jacobi iteration (device driver; that iteration always does work)
+  static constexpr int kRing = 64, kEv = 8;
   hipEvent_t ev[kRing * kEv] = {};
+  bool kev_rec[kRing] = {};
+  double k_sum[2] = {0, 0};
+  long k_steps = 0;
   long t_recorded = 0, t_processed = 0;
   double t_sum[kEv] = {0};
   int persistent_grid = 0;
@@ -718,7 +723,31 @@ static void timing_fold_one(wsph_solver *s) {
       s->t_sum[k] += ms;
   }
   if (hipEventElapsedTime(&ms, s->ev[base], s->ev[base + 4]) == hipSuccess) s->t_sum[4] += ms;
+  const int slot = (int)(s->t_processed % wsph_solver::kRing);
+  if (s->kev_rec[slot]) {
+    float m1, m2;
+    if (hipEventElapsedTime(&m1, s->ev[base + 5], s->ev[base + 6]) == hipSuccess &&
+        hipEventElapsedTime(&m2, s->ev[base + 6], s->ev[base + 7]) == hipSuccess) {
+      s->k_sum[0] += m1;
+      s->k_sum[1] += m2;
+      s->k_steps++;
+    }
+    s->kev_rec[slot] = false;
+  }
   s->t_processed++;
+}
+
+// Events around the first Jacobi iteration's J1 and J2 of the step being
+// recorded (null when timing is off).
+static void kernel_events(wsph_solver *s, hipEvent_t ev3[3]) {
+  if (!s->timing) {
+    ev3[0] = ev3[1] = ev3[2] = nullptr;
+    return;
+  }
+  const int slot = (int)(s->t_recorded % wsph_solver::kRing);
+  const int base = slot * wsph_solver::kEv;
+  for (int k = 0; k < 3; ++k) ev3[k] = s->ev[base + 5 + k];
+  s->kev_rec[slot] = true;
 }
 
 int wsph_neighbor_search(wsph_solver *s) {
@@ -778,7 +807,9 @@ static int device_loop(wsph_solver *s, const Bufs &b, int src, int queued, int m
   const int tag = s->step_count;
   for (int m = m0; queued < max_it; ++m) {
     const int cnt = C < max_it - queued ? C : max_it - queued;
-    launch_jacobi_chunk(b, s->C, P, src, cnt, P.block, s->stream);
+    hipEvent_t ev3[3] = {nullptr, nullptr, nullptr};
+    if (queued == 0) kernel_events(s, ev3);
+    launch_jacobi_chunk(b, s->C, P, src, cnt, P.block, s->stream, ev3);
     queued += cnt;
     HIP_TRY(hipMemcpyAsync(&s->done_flags[m], &s->st->done_tag, sizeof(int),
                            hipMemcpyDeviceToHost, s->stream));
@@ -1313,15 +1344,19 @@ int wsph_set_timing(wsph_solver *s, int enable) {
   s->timing = enable != 0;
   s->t_recorded = s->t_processed = 0;
   for (double &v : s->t_sum) v = 0.0;
+  s->k_sum[0] = s->k_sum[1] = 0.0;
+  s->k_steps = 0;
+  for (bool &f : s->kev_rec) f = false;
   return WSPH_OK;
 }
 
 int wsph_last_timings(wsph_solver *s, double *out, int nout) {
   if (!s || !out) return fail(WSPH_E_ARG, "null argument");
   while (s->t_processed < s->t_recorded) timing_fold_one(s);
-  double v[8] = {s->t_sum[0], s->t_sum[1], s->t_sum[2], s->t_sum[3], s->t_sum[4],
-                 (double)s->persistent_grid, (double)s->t_processed, 0.0};
-  for (int k = 0; k < nout && k < 8; ++k) out[k] = v[k];
+  double v[10] = {s->t_sum[0], s->t_sum[1], s->t_sum[2], s->t_sum[3], s->t_sum[4],
+                  (double)s->persistent_grid, (double)s->t_processed, s->k_sum[0], s->k_sum[1],
+                  (double)s->k_steps};
+  for (int k = 0; k < nout && k < 10; ++k) out[k] = v[k];
   return WSPH_OK;
 }
 

@@ -102,9 +102,11 @@ void launch_learned_start(const Bufs &b, const SphConst &c, const Weights *net, 
 // accumulates the residual of iteration `slot`.
 void launch_j1(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s);
 void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, hipStream_t s);
-// `iters` Jacobi iterations with device-side stop test (no-ops once done)
+// `iters` Jacobi iterations with device-side stop test (no-ops once done);
+// ev3 (optional): events recorded before J1, between J1 and J2, after J2 of
+// the chunk's first iteration
 void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P, int src0,
-                         int iters, int block, hipStream_t s);
+                         int iters, int block, hipStream_t s, hipEvent_t *ev3 = nullptr);
 // src < 0: read the final buffer index from DevState
 void launch_integrate(const Bufs &b, const SphConst &c, int src, int start_mode, int block,
                       hipStream_t s);

@@ -1191,11 +1191,15 @@ void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, h
 }
 
 void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P, int src0,
-                         int iters, int block, hipStream_t s) {
+                         int iters, int block, hipStream_t s, hipEvent_t *ev3) {
   if (b.n == 0) return;
   for (int k = 0; k < iters; ++k) {
+    const bool ev = ev3 && ev3[0] && k == 0;
+    if (ev) (void)hipEventRecord(ev3[0], s);
     DISPATCH_LDS(k_j1_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0);
+    if (ev) (void)hipEventRecord(ev3[1], s);
     DISPATCH_LDS(k_j2_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0);
+    if (ev) (void)hipEventRecord(ev3[2], s);
     if (b.scenes)
       k_batch_stop<<<1, 1024, 0, s>>>(b, src0, P.min_iters, P.max_iters, P.force_iters, P.eta);
     else

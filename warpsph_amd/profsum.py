@@ -1,6 +1,7 @@
 """Summaries of rocprofv3 result databases (rocpd SQLite).
 
     python -m warpsph_amd.profsum kernels <results.db> [--steps N]
+    python -m warpsph_amd.profsum dispatches <kernel_trace.csv> [--steps N]
     python -m warpsph_amd.profsum traffic <fetch.db> <write.db> <copy_fetch.db> <copy_write.db> [--copy-bytes B]
 
 ``kernels`` prints per-kernel call counts and durations (the --stats view);
@@ -26,6 +27,37 @@ def kernels(db: str, steps: int | None = None, limit: int = 25) -> str:
         line = f"{name[:64]:64s} {calls:6d} {tot:11.1f} {avg:9.2f} {pct:6.2f}"
         if steps:
             line += f"  {tot / steps:8.1f}"
+        out.append(line)
+    return "\n".join(out)
+
+
+def dispatches(csv_path: str, steps: int | None = None, limit: int = 25) -> str:
+    """Per-kernel durations from a rocprofv3 kernel-trace CSV, with the mean
+    over working dispatches beside the plain mean: the device-driven Jacobi
+    loop also issues launches that return at once after its stop test, and a
+    dispatch counts as working when it takes more than a quarter of the
+    kernel's 90th-percentile duration."""
+    import csv
+    per = collections.defaultdict(list)
+    with open(csv_path, newline="") as fh:
+        rows = csv.DictReader(fh)
+        for r in rows:
+            keys = {k.lower(): k for k in r}
+            name = r[keys.get("kernel_name", "kernel_name")]
+            t0 = int(r[keys["start_timestamp"]])
+            t1 = int(r[keys["end_timestamp"]])
+            per[name].append((t1 - t0) / 1000.0)
+    total = sum(sum(v) for v in per.values())
+    out = [f"{'kernel':60s} {'calls':>6s} {'total_us':>11s} {'avg_us':>9s} {'work':>6s} "
+           f"{'work_avg_us':>11s} {'%':>6s}" + ("  us/step" if steps else "")]
+    for name, ds in sorted(per.items(), key=lambda kv: -sum(kv[1]))[:limit]:
+        srt = sorted(ds)
+        p90 = srt[int(0.9 * (len(srt) - 1))]
+        work = [d for d in ds if d > 0.25 * p90]
+        line = (f"{name.split('(')[0][:60]:60s} {len(ds):6d} {sum(ds):11.1f} {sum(ds) / len(ds):9.2f} "
+                f"{len(work):6d} {sum(work) / max(1, len(work)):11.2f} {100 * sum(ds) / total:6.2f}")
+        if steps:
+            line += f"  {sum(ds) / steps:8.1f}"
         out.append(line)
     return "\n".join(out)
 
@@ -81,6 +113,9 @@ def main():
     k = sub.add_parser("kernels")
     k.add_argument("db")
     k.add_argument("--steps", type=int)
+    dsp = sub.add_parser("dispatches")
+    dsp.add_argument("csv")
+    dsp.add_argument("--steps", type=int)
     t = sub.add_parser("traffic")
     t.add_argument("fetch")
     t.add_argument("write")
@@ -90,6 +125,8 @@ def main():
     a = ap.parse_args()
     if a.cmd == "kernels":
         print(kernels(a.db, a.steps))
+    elif a.cmd == "dispatches":
+        print(dispatches(a.csv, a.steps))
     else:
         print(json.dumps(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes),
                          indent=1))
