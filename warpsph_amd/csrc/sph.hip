@@ -159,34 +159,124 @@ __device__ __forceinline__ void for_span(const float4 *__restrict__ pts, int s, 
   for (; j < e; ++j) test(j, pts[j]);
 }
 
-__global__ void k_lists(Bufs b, SphConst c) {
-  const int i = particle_index();
-  int nf = 0, ns = 0;
-  if (i < b.n) {
-    const float4 p = b.pos[i];
-    const int n = b.n;
-    const int scene = b.scenes ? b.sid[i] : 0;
-    const GridRef fg = fluid_grid(b, c, scene);
-    for_query_spans(p.x, p.y, p.z, c, fg.dims, fg.start, fg.count, [&](int s0, int e0) {
-      for_span(b.pos, s0, e0, [&](int j, const float4 &o) {
-        // distance < SMOOTHING_LENGTH and index != i (IISPH.py:72)
-        if (j != i && dist2(p.x, p.y, p.z, o.x, o.y, o.z) < c.r2_lt) {
-          if (nf < b.cap_ff) b.nbr_ff[(long)nf * n + i] = j;
-          ++nf;
-        }
-      });
-    });
-    const GridRef bg = boundary_grid(b, c, scene);
-    if (b.nb > 0)
-      for_query_spans(p.x, p.y, p.z, c, bg.dims, bg.start, bg.count, [&](int s0, int e0) {
-        for_span(b.bpos, s0, e0, [&](int j, const float4 &o) {
-          // distance < SMOOTHING_LENGTH (IISPH.py:129)
-          if (dist2(p.x, p.y, p.z, o.x, o.y, o.z) < c.r2_lt) {
-            if (ns < b.cap_fs) b.nbr_fs[(long)ns * n + i] = j;
-            ++ns;
+// The same traversal as for_query_spans with wave-uniform loop counts: every
+// lane runs as many (z, y) rows as the widest query range of its wave (rows
+// outside a lane's own range visit nothing), and `row_done` runs after each
+// row with the whole wave converged. Lanes with live == false visit nothing.
+template <typename VisitF, typename RowF>
+__device__ __forceinline__ void for_query_rows(float px, float py, float pz, bool live,
+                                               const SphConst &c, const int *dims,
+                                               const unsigned *__restrict__ start,
+                                               const unsigned *__restrict__ count, VisitF visit,
+                                               RowF row_done) {
+  const Range3 q = query_range(px, py, pz, c.h, c.cell_inv, dims);
+  const int nz = wave_max_i32(live ? q.z1 - q.z0 + 1 : 0);
+  const int ny = wave_max_i32(live ? q.y1 - q.y0 + 1 : 0);
+  for (int dz = 0; dz < nz; ++dz) {
+    const int z = q.z0 + dz;
+    for (int dy = 0; dy < ny; ++dy) {
+      const int y = q.y0 + dy;
+      if (live && z <= q.z1 && y <= q.y1) {
+        const int hz = hash_axis(z, dims[2]);
+        const int hy = hash_axis(y, dims[1]);
+        const int hx0 = hash_axis(q.x0, dims[0]);
+        const int hx1 = hash_axis(q.x1, dims[0]);
+        if (c.layout == WSPH_LAYOUT_LINEAR && hx1 - hx0 == q.x1 - q.x0) {
+          const int s0 = cell_slot(hx0, hy, hz, dims, c.layout);
+          const int s1 = s0 + (hx1 - hx0);
+          visit((int)start[s0], (int)(start[s1] + count[s1]));
+        } else {
+          for (int x = q.x0; x <= q.x1; ++x) {
+            const int sl = cell_slot(hash_axis(x, dims[0]), hy, hz, dims, c.layout);
+            visit((int)start[sl], (int)(start[sl] + count[sl]));
           }
+        }
+      }
+      row_done();
+    }
+  }
+}
+
+// Neighbour list writer of one wave. Entry k of lane l belongs at
+// list[k * n + i_l]; the lanes find their entries at different times, and
+// storing each as found fills a 128-byte line of slot k from ~32 separate
+// partial stores, between which the line is evicted and written back again
+// under the candidate-read traffic. Entries are therefore kept in a ring of
+// kListRing slot rows per wave in LDS, and a slot row is stored once, as one
+// coalesced row, when every lane of the wave has passed it (after each query
+// row) or at the end. A lane more than kListRing slots ahead of the wave's
+// flushed rows stores its later entries directly.
+constexpr int kListRing = 16;
+struct ListWriter {
+  int *ring;      // [kListRing][64] of this wave
+  int *list;
+  long n;
+  int i, lane, cap;
+  bool live;
+  int cnt = 0, direct = 0x7fffffff, flushed = 0;
+  __device__ void put(int j) {
+    if (cnt < cap) {
+      if (direct == 0x7fffffff && cnt - flushed < kListRing) {
+        ring[(cnt % kListRing) * 64 + lane] = j;
+      } else {
+        if (direct == 0x7fffffff) direct = cnt;
+        list[(long)cnt * n + i] = j;
+      }
+    }
+    ++cnt;
+  }
+  __device__ void flush_to(int upto) {  // wave-uniform
+    for (int k = flushed; k < upto; ++k)
+      if (live && k < cnt && k < cap && k < direct) list[(long)k * n + i] = ring[(k % kListRing) * 64 + lane];
+    flushed = upto > flushed ? upto : flushed;
+  }
+  __device__ void row_done() {  // wave-uniform: rows every lane has completed
+    int m = live ? cnt : 0x7fffffff;
+    for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
+    m = m < cap ? m : cap;
+    if (m > flushed) flush_to(m);
+  }
+  __device__ void finish() {
+    const int m = wave_max_i32(live ? (cnt < cap ? cnt : cap) : 0);
+    flush_to(m);
+  }
+};
+
+__global__ void k_lists(Bufs b, SphConst c) {
+  int *ring = reinterpret_cast<int *>(g_smem) + (threadIdx.x >> 6) * (kListRing * 64);
+  const int i = particle_index();
+  const bool live = i < b.n;
+  const int n = b.n;
+  const float4 p = live ? b.pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const int scene = b.scenes && live ? b.sid[i] : 0;
+  ListWriter wf{ring, b.nbr_ff, n, i, (int)(threadIdx.x & 63), b.cap_ff, live};
+  const GridRef fg = fluid_grid(b, c, scene);
+  for_query_rows(
+      p.x, p.y, p.z, live, c, fg.dims, fg.start, fg.count,
+      [&](int s0, int e0) {
+        for_span(b.pos, s0, e0, [&](int j, const float4 &o) {
+          // distance < SMOOTHING_LENGTH and index != i (IISPH.py:72)
+          if (j != i && dist2(p.x, p.y, p.z, o.x, o.y, o.z) < c.r2_lt) wf.put(j);
         });
-      });
+      },
+      [&]() { wf.row_done(); });
+  wf.finish();
+  ListWriter wb{ring, b.nbr_fs, n, i, (int)(threadIdx.x & 63), b.cap_fs, live};
+  if (b.nb > 0) {
+    const GridRef bg = boundary_grid(b, c, scene);
+    for_query_rows(
+        p.x, p.y, p.z, live, c, bg.dims, bg.start, bg.count,
+        [&](int s0, int e0) {
+          for_span(b.bpos, s0, e0, [&](int j, const float4 &o) {
+            // distance < SMOOTHING_LENGTH (IISPH.py:129)
+            if (dist2(p.x, p.y, p.z, o.x, o.y, o.z) < c.r2_lt) wb.put(j);
+          });
+        },
+        [&]() { wb.row_done(); });
+    wb.finish();
+  }
+  const int nf = wf.cnt, ns = wb.cnt;
+  if (live) {
     b.cnt_ff[i] = nf < b.cap_ff ? nf : b.cap_ff;
     b.cnt_fs[i] = ns < b.cap_fs ? ns : b.cap_fs;
   }
@@ -1160,7 +1250,8 @@ void launch_boundary_psi(const Bufs &b, const SphConst &c, hipStream_t s) {
 
 void launch_lists(const Bufs &b, const SphConst &c, int block, hipStream_t s) {
   if (b.n == 0) return;
-  k_lists<<<nblocks(b.n, block), block, 0, s>>>(b, c);
+  const size_t ring = sizeof(int) * (size_t)(block / 64) * kListRing * 64;
+  k_lists<<<nblocks(b.n, block), block, ring, s>>>(b, c);
 }
 
 void launch_density(const Bufs &b, const SphConst &c, bool fuse_vadv, int block, hipStream_t s) {
