@@ -69,7 +69,7 @@ def parse():
     ap.add_argument("--trace-steps", type=int, default=0,
                     help="print wall time and solver statistics of this many single steps first")
     ap.add_argument("--autotune", choices=["off", "cached", "force"], default="cached",
-                    help="batch workload: launch shapes from the autotuner (cache or fresh sweep)")
+                    help="launch shapes from the autotuner (its cache, or a fresh sweep)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -291,6 +291,22 @@ def main():
             raise SystemExit(f"{args.net} is missing: train it with "
                              "`python -m warpsph_amd.train_predictor --out {args.net}`")
         net = PressureNet.load(args.net, max_batch=1024)
+    tuned = None
+    if args.autotune != "off":
+        from warpsph_amd import autotune
+        from warpsph_amd.solver import IISPH
+        key = f"{args.preset}|start{WARM[args.warm_start]}|{args.jacobi}"
+        tune_cfg = base.replace(warm_start=WARM[args.warm_start])
+        cands = None
+        if args.jacobi != "device":
+            cands = [c for c in autotune.candidates() if c["jacobi_mode"] == tune_cfg.jacobi_mode]
+        from warpsph_amd import scene as scn
+        sc = scn.build(tune_cfg)
+        tuned = autotune.tune(lambda c: IISPH(stage_path=None, config=c, scene=sc, predictor=net),
+                              tune_cfg,
+                              key=key, cands=cands, force=args.autotune == "force",
+                              log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+        base = autotune.apply(base, tuned["shape"])
     cfg = base.replace(warm_start=WARM[args.warm_start])
     r = timed_run(cfg, net, args, trace=True)
     s, n, elapsed, steps, tm = r["solver"], r["n"], r["elapsed"], r["steps"], r["tm"]
@@ -336,6 +352,12 @@ def main():
         "step_roofline_frac": (n * (BYTES_PER_STEP_FIXED * steps + BYTES_PER_AP_EVAL * ap_evals)
                                / elapsed / 1e9) / HBM_PEAK_GBS,
     }
+    line["config"]["shape"] = {k: getattr(cfg, k) for k in ("block", "jacobi_mode", "chunk_iters",
+                                                            "coop_blocks_per_cu", "cell_layout",
+                                                            "graph_iters")}
+    if tuned is not None:
+        line["autotune"] = {"cached": tuned["cached"], "ms_per_step": tuned["ms_per_step"],
+                            "candidates": len(tuned.get("table", []))}
     if net is not None:
         h = s.history(args.steps)
         line["learned"] = {
