@@ -9,8 +9,8 @@
 //      particle and its fluid neighbours (sum_j W_ij f_j / sum_j W_ij, the
 //      particle itself entering with W(0)),
 //   2. optionally writes the 8-feature row (training pairs),
-//   3. stages the rows of its 64 particles in LDS and runs the network on
-//      them as two 32-row MFMA tiles (BF16 operands, FP32 accumulation),
+//   3. runs the network on the wave's 64 rows in registers (infer64: two
+//      32-row MFMA tiles, BF16 operands, FP32 accumulation),
 // A network switched off by the judge is evaluated every kRetryEvery-th step
 // only (DevState::net_eval records whether it ran).
 //   4. writes the start value p0 = max(0, carry + P_ref * y) (the carried
@@ -31,20 +31,20 @@ struct PosSeed {
 };
 
 template <bool kInfer>
-__global__ __launch_bounds__(64 * kLearnWaves) void k_learned_start(Bufs b, SphConst c, Weights W) {
-  __shared__ float xf[kLearnWaves][64][WSPH_MLP_IN];
-  __shared__ float act[kInfer ? kLearnWaves : 1][32][65];
-  __shared__ float ys[kLearnWaves][64];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+__global__ __launch_bounds__(64 * kLearnWaves, 4) void k_learned_start(Bufs b, SphConst c, Weights W) {
+  __shared__ NetLds L;
+  bool eval = false;
   if constexpr (kInfer) {
     // a switched-off network is retried every kRetryEvery-th step (the same
     // decision in every block); without logging nothing else is needed
-    const bool eval = !b.st->net_off || b.st->step_count % kRetryEvery == 0;
+    eval = !b.st->net_off || b.st->step_count % kRetryEvery == 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) b.st->net_eval = eval ? 1 : 0;
     if (!eval && !b.feat) return;
+    if (eval) stage_net(L, W);
+    __syncthreads();
   }
   const int i = particle_index();
-  const bool live = i < b.n;  // no early exit below: forward_tile has block barriers
+  const bool live = i < b.n;  // whole waves stay to the end: infer64 exchanges lanes
   float f[WSPH_MLP_IN] = {};
   float carry = 0.0f, a_ii = 0.0f;
   if (live) {
@@ -84,21 +84,10 @@ __global__ __launch_bounds__(64 * kLearnWaves) void k_learned_start(Bufs b, SphC
     }
   }
   if constexpr (kInfer) {
+    if (!eval) return;
     const int net_off = b.st->net_off;
-#pragma unroll
-    for (int k = 0; k < WSPH_MLP_IN; ++k) xf[wave][lane][k] = f[k];
-    __syncthreads();
-#pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      float y[16];
-      forward_tile(W, &xf[wave][32 * half][0], 32, act[wave], nullptr, nullptr, y);
-      if ((lane & 31) == 0)
-#pragma unroll
-        for (int k = 0; k < 16; ++k) ys[wave][32 * half + drow(lane, k)] = y[k];
-    }
-    __syncthreads();
+    const float y = infer64(L, f);
     if (live) {
-      const float y = ys[wave][lane];
       float pn = isfinite(y) ? carry + c.pref * y : carry;
       pn = pn > 0.0f ? pn : 0.0f;
       if (fabsf(f[1]) < kWeakCoupling) pn = carry;

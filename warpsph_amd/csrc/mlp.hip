@@ -31,27 +31,23 @@ __global__ void k_to_bf16(const float *__restrict__ p, __bf16 *__restrict__ w1,
 
 
 // ---- inference on a feature matrix -------------------------------------------
+// 64 rows per wave and trip, register-resident (infer64).
 constexpr int kMlpWaves = 4;
 
 __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_forward(Weights W, const float *__restrict__ x,
                                                                 int n, float *__restrict__ y) {
-  __shared__ float act[kMlpWaves][32][65];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int tiles = (n + 31) / 32;
-  // every wave of the block runs the same number of loop trips (the tile
-  // code has block barriers); waves past the last tile work on no rows
-  for (int t0 = blockIdx.x * kMlpWaves; t0 < tiles; t0 += gridDim.x * kMlpWaves) {
-    const int t = t0 + wave;
-    const bool live = t < tiles;
-    const int tt = live ? t : tiles - 1;
-    const int rows = live ? min(32, n - 32 * tt) : 0;
-    float out[16];
-    forward_tile(W, x + (long)32 * tt * WSPH_MLP_IN, rows, act[wave], nullptr, nullptr, out);
-    if (live && (lane & 31) == 0)
-      for (int i = 0; i < 16; ++i) {
-        const int r = drow(lane, i);
-        if (r < rows) y[(long)32 * tt + r] = out[i];
-      }
+  __shared__ NetLds L;
+  stage_net(L, W);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const long waves = (long)gridDim.x * kMlpWaves;
+  for (long r0 = ((long)blockIdx.x * kMlpWaves + (threadIdx.x >> 6)) * 64; r0 < n; r0 += waves * 64) {
+    const long row = r0 + lane;
+    float f[WSPH_MLP_IN];
+#pragma unroll
+    for (int k = 0; k < WSPH_MLP_IN; ++k) f[k] = row < n ? x[row * WSPH_MLP_IN + k] : 0.0f;
+    const float v = infer64(L, f);
+    if (row < n) y[row] = v;
   }
 }
 
@@ -275,7 +271,7 @@ void mlp_sync_weights(const MlpDev &m, hipStream_t s) {
 
 void mlp_forward(const MlpDev &m, const float *x, int n, float *y, hipStream_t s) {
   if (n <= 0) return;
-  const int tiles = (n + 31) / 32;
+  const int tiles = (n + 63) / 64;
   int blocks = (tiles + kMlpWaves - 1) / kMlpWaves;
   if (blocks > 4096) blocks = 4096;
   k_mlp_forward<<<blocks, 64 * kMlpWaves, 0, s>>>(weights_of(m), x, n, y);

@@ -127,6 +127,127 @@ __device__ __forceinline__ void forward_tile(const Weights &W, const float *x, i
   __syncthreads();
 }
 
+// ---- register-resident inference of 64 rows per wave --------------------
+// The transposed products z^T = W x^T are computed (M = hidden units,
+// N = the wave's rows), so that a layer's accumulator, in D layout, becomes
+// the next layer's B operand with one exchange of four values between lanes
+// l and l ^ 32 per 16-wide k-slice: no LDS round trip and no block barrier.
+// The weights sit in LDS (staged once per block), each lane reading its A
+// operand rows as 16-byte loads; the output layer is a dot product on the
+// VALU (bf16 operands, exact products, FP32 sums, like the matrix cores).
+constexpr int kW2Pitch = WSPH_MLP_H + 8;  // bf16 row pitch of W2 in LDS (breaks bank conflicts)
+struct NetLds {
+  __bf16 w1[WSPH_MLP_H * WSPH_MLP_IN];
+  __bf16 w2[WSPH_MLP_H * kW2Pitch];
+  float b1[WSPH_MLP_H], b2[WSPH_MLP_H], w3[WSPH_MLP_H];  // w3 holds the bf16 values
+  float b3;
+};
+
+// Every thread of the block calls this once; a barrier follows.
+__device__ __forceinline__ void stage_net(NetLds &L, const Weights &W) {
+  for (int k = threadIdx.x; k < WSPH_MLP_H * WSPH_MLP_H; k += blockDim.x) {
+    L.w2[(k / WSPH_MLP_H) * kW2Pitch + k % WSPH_MLP_H] = W.w2[k];
+    if (k < WSPH_MLP_H * WSPH_MLP_IN) L.w1[k] = W.w1[k];
+    if (k < WSPH_MLP_H) {
+      L.b1[k] = W.b1[k];
+      L.b2[k] = W.b2[k];
+      L.w3[k] = (float)W.w3[k];
+    }
+  }
+  if (threadIdx.x == 0) L.b3 = W.b3[0];
+}
+
+// D-layout bias of M-tile t: register i of lane (n, h) holds unit
+// m = 32 t + 8 (i / 4) + 4 h + i % 4.
+__device__ __forceinline__ f32x16 bias_d(const float *b, int t, int h) {
+  f32x16 c;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const float4 v = *reinterpret_cast<const float4 *>(b + 32 * t + 8 * g + 4 * h);
+    c[4 * g] = v.x;
+    c[4 * g + 1] = v.y;
+    c[4 * g + 2] = v.z;
+    c[4 * g + 3] = v.w;
+  }
+  return c;
+}
+
+// y for the row of every lane of the wave (lane l holds row l's 8 features
+// in f). Rows 0-31 form the first tile, rows 32-63 the second.
+__device__ __forceinline__ float infer64(const NetLds &L, const float f[WSPH_MLP_IN]) {
+  const int lane = threadIdx.x & 63, h = lane >> 5, r = lane & 31;
+  float yt[2];
+  // one tile, and within it one accumulator, live at a time (register budget)
+#pragma unroll 1
+  for (int tile = 0; tile < 2; ++tile) {
+    // an offset the compiler cannot see through: the weight loads stay in
+    // this iteration instead of being hoisted (and kept live) across tiles
+    int salt = 0;
+    asm volatile("" : "+v"(salt));
+    const __bf16 *w1 = L.w1 + salt, *w2 = L.w2 + salt;
+    const float *b1 = L.b1 + salt, *b2 = L.b2 + salt, *w3 = L.w3 + salt;
+    // B of layer 1: x^T[k][n], lanes 0-31 hold row n's features, 32-63 the
+    // zero padding of K = 8 to 16
+    bf16x8 bx;
+#pragma unroll
+    for (int e = 0; e < WSPH_MLP_IN; ++e) {
+      const float v = tile == 0 ? f[e] : __shfl_xor(f[e], 32, 64);
+      bx[e] = h == 0 ? (__bf16)v : (__bf16)0.0f;
+    }
+    // layer 1, M-tile by M-tile; its activations (D layout) become the B
+    // operands of layer 2's k-slices 2t and 2t + 1
+    bf16x8 bh[4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      bf16x8 aw;
+      if (h == 0) {
+        aw = *reinterpret_cast<const bf16x8 *>(&w1[(32 * t + r) * WSPH_MLP_IN]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) aw[e] = (__bf16)0.0f;
+      }
+      const f32x16 d = mfma(aw, bx, bias_d(b1, t, h));
+      float a[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) a[i] = silu(d[i]);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int g0 = 2 * q;
+        float recv[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) recv[e] = __shfl_xor(h == 0 ? a[4 * (g0 + 1) + e] : a[4 * g0 + e], 32, 64);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          bh[2 * t + q][e] = (__bf16)(h == 0 ? a[4 * g0 + e] : recv[e]);
+          bh[2 * t + q][e + 4] = (__bf16)(h == 0 ? recv[e] : a[4 * (g0 + 1) + e]);
+        }
+      }
+    }
+    // layer 2, M-tile by M-tile, folded into the output layer's dot product
+    float part = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      f32x16 d = bias_d(b2, t, h);
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const bf16x8 aw = *reinterpret_cast<const bf16x8 *>(
+            &w2[(32 * t + r) * kW2Pitch + 16 * ks + 8 * h]);
+        d = mfma(aw, bh[ks], d);
+      }
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const float4 w = *reinterpret_cast<const float4 *>(w3 + 32 * t + 8 * g + 4 * h);
+        part += (float)(__bf16)silu(d[4 * g]) * w.x;
+        part += (float)(__bf16)silu(d[4 * g + 1]) * w.y;
+        part += (float)(__bf16)silu(d[4 * g + 2]) * w.z;
+        part += (float)(__bf16)silu(d[4 * g + 3]) * w.w;
+      }
+    }
+    yt[tile] = part + __shfl_xor(part, 32, 64) + L.b3;
+  }
+  return h == 0 ? yt[0] : yt[1];
+}
+
 inline Weights weights_of(const MlpDev &m) {
   Weights W;
   W.w1 = m.w1;
