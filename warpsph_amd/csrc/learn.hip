@@ -30,9 +30,12 @@ struct PosSeed {
   float4 pos, seed;
 };
 
+constexpr int kLearnTable = 4096;  // W table entries kept in LDS (larger tables: global)
+
 template <bool kInfer>
 __global__ __launch_bounds__(64 * kLearnWaves, 4) void k_learned_start(Bufs b, SphConst c, Weights W) {
   __shared__ NetLds L;
+  __shared__ float wt_lds[kLearnTable];
   bool eval = false;
   if constexpr (kInfer) {
     // a switched-off network is retried every kRetryEvery-th step (the same
@@ -41,8 +44,13 @@ __global__ __launch_bounds__(64 * kLearnWaves, 4) void k_learned_start(Bufs b, S
     if (blockIdx.x == 0 && threadIdx.x == 0) b.st->net_eval = eval ? 1 : 0;
     if (!eval && !b.feat) return;
     if (eval) stage_net(L, W);
-    __syncthreads();
   }
+  // the kernel table in LDS: each lane's lookups hit other addresses
+  const bool tab_lds = c.nw <= kLearnTable;
+  if (tab_lds)
+    for (int k = threadIdx.x; k < c.nw; k += blockDim.x) wt_lds[k] = b.wt[k];
+  __syncthreads();
+  const float *wtab = tab_lds ? wt_lds : b.wt;
   const int i = particle_index();
   const bool live = i < b.n;  // whole waves stay to the end: infer64 exchanges lanes
   float f[WSPH_MLP_IN] = {};
@@ -51,14 +59,14 @@ __global__ __launch_bounds__(64 * kLearnWaves, 4) void k_learned_start(Bufs b, S
     const int n = b.n;
     const float4 p = b.pos[i];
     const float4 s = b.seed[i];
-    const float w0 = spline_w(b.wt, c.nw, c.w_scale, c.h, 0.0f);
+    const float w0 = spline_w(wtab, c.nw, c.w_scale, c.h, 0.0f);
     float sw = w0, m0 = w0 * s.x, m1 = w0 * s.y, m2 = w0 * s.z, m3 = w0 * s.w;
     const float4 *__restrict__ pos = b.pos;
     const float4 *__restrict__ seed = b.seed;
     for_neighbors<4>(
         b.nbr_ff, n, i, b.cnt_ff[i], [&](int j, long) { return PosSeed{pos[j], seed[j]}; },
         [&](const PosSeed &v) {
-          const float w = spline_w(b.wt, c.nw, c.w_scale, c.h,
+          const float w = spline_w(wtab, c.nw, c.w_scale, c.h,
                                    dist3(p.x, p.y, p.z, v.pos.x, v.pos.y, v.pos.z));
           sw += w;
           m0 += w * v.seed.x;

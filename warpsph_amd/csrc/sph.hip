@@ -159,10 +159,41 @@ __device__ __forceinline__ void for_span(const float4 *__restrict__ pts, int s, 
   for (; j < e; ++j) test(j, pts[j]);
 }
 
-// The same traversal as for_query_spans with wave-uniform loop counts: every
-// lane runs as many (z, y) rows as the widest query range of its wave (rows
-// outside a lane's own range visit nothing), and `row_done` runs after each
-// row with the whole wave converged. Lanes with live == false visit nothing.
+// hash_axis with the modulo as a float-reciprocal division: for
+// 0 <= c < 2^22 and 1 <= dim <= 1024, q = (int)((float)c * (1.0f / dim)) is
+// off by at most one, which one correction in each direction removes (checked
+// exhaustively over that whole range); larger c keep the integer modulo.
+__device__ __forceinline__ int hash_axis_fast(int c, int dim, float inv) {
+  c += 1 << 20;
+  c = c < 0 ? 0 : c;
+  if (c >= (1 << 22)) return c % dim;
+  const int q = (int)((float)c * inv);
+  int r = c - q * dim;
+  r = r < 0 ? r + dim : r;
+  return r >= dim ? r - dim : r;
+}
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// a[k] for k in [0, 4) by selects (no indexed register array, which would
+// live in scratch memory inside a loop that is not unrolled)
+__device__ __forceinline__ uint32_t pick4(const uint32_t (&a)[4], int k) {
+  return k == 0 ? a[0] : k == 1 ? a[1] : k == 2 ? a[2] : a[3];
+}
+
+// The traversal of for_query_spans (same cells, same order) with wave-uniform
+// loop counts: every lane runs as many (z, y) rows as the widest query range
+// of its wave (rows outside a lane's own range visit nothing), and `row_done`
+// runs after each row with the whole wave converged. The hashed coordinates
+// of the (at most 4) cells per axis and their Morton bit spreads are computed
+// once per query. Lanes with live == false visit nothing.
 template <typename VisitF, typename RowF>
 __device__ __forceinline__ void for_query_rows(float px, float py, float pz, bool live,
                                                const SphConst &c, const int *dims,
@@ -170,24 +201,36 @@ __device__ __forceinline__ void for_query_rows(float px, float py, float pz, boo
                                                const unsigned *__restrict__ count, VisitF visit,
                                                RowF row_done) {
   const Range3 q = query_range(px, py, pz, c.h, c.cell_inv, dims);
+  const int d0 = dims[0], d1 = dims[1], d2 = dims[2];
+  const float i0 = 1.0f / (float)d0, i1 = 1.0f / (float)d1, i2 = 1.0f / (float)d2;
+  const bool morton = c.layout == WSPH_LAYOUT_MORTON;
+  uint32_t kx[4], ky[4], kz[4];  // per-axis slot parts of the range's cells
+  const int hx0 = hash_axis_fast(q.x0, d0, i0), hx1 = hash_axis_fast(q.x1, d0, i0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int hx = hash_axis_fast(q.x0 + k, d0, i0);
+    const int hy = hash_axis_fast(q.y0 + k, d1, i1);
+    const int hz = hash_axis_fast(q.z0 + k, d2, i2);
+    kx[k] = morton ? spread3((uint32_t)hx) : (uint32_t)hx;
+    ky[k] = morton ? spread3((uint32_t)hy) << 1 : (uint32_t)(hy * d0);
+    kz[k] = morton ? spread3((uint32_t)hz) << 2 : (uint32_t)(hz * d0 * d1);
+  }
+  const int nxl = q.x1 - q.x0 + 1;
   const int nz = wave_max_i32(live ? q.z1 - q.z0 + 1 : 0);
   const int ny = wave_max_i32(live ? q.y1 - q.y0 + 1 : 0);
   for (int dz = 0; dz < nz; ++dz) {
-    const int z = q.z0 + dz;
     for (int dy = 0; dy < ny; ++dy) {
-      const int y = q.y0 + dy;
-      if (live && z <= q.z1 && y <= q.y1) {
-        const int hz = hash_axis(z, dims[2]);
-        const int hy = hash_axis(y, dims[1]);
-        const int hx0 = hash_axis(q.x0, dims[0]);
-        const int hx1 = hash_axis(q.x1, dims[0]);
-        if (c.layout == WSPH_LAYOUT_LINEAR && hx1 - hx0 == q.x1 - q.x0) {
-          const int s0 = cell_slot(hx0, hy, hz, dims, c.layout);
-          const int s1 = s0 + (hx1 - hx0);
+      if (live && q.z0 + dz <= q.z1 && q.y0 + dy <= q.y1) {
+        const uint32_t zy = pick4(kz, dz) + pick4(ky, dy);
+        if (!morton && hx1 - hx0 == nxl - 1) {
+          // linear layout, no wrap: the row's cells are consecutive slots
+          const int s0 = (int)(zy + kx[0]);
+          const int s1 = s0 + (nxl - 1);
           visit((int)start[s0], (int)(start[s1] + count[s1]));
         } else {
-          for (int x = q.x0; x <= q.x1; ++x) {
-            const int sl = cell_slot(hash_axis(x, dims[0]), hy, hz, dims, c.layout);
+          for (int dx = 0; dx < nxl; ++dx) {
+            const uint32_t x = pick4(kx, dx);
+            const int sl = (int)(morton ? (zy | x) : zy + x);
             visit((int)start[sl], (int)(start[sl] + count[sl]));
           }
         }
