@@ -100,19 +100,9 @@ __global__ void k_solve_result(DevState *st, int iters, double residual, int con
   st->converged = converged;
   st->ap_evals += ap_evals;
 }
-__global__ void k_add_ap_evals(DevState *st, int k) { st->ap_evals += k; }
 __global__ void k_set_dt(DevState *st, double dt, double t) {
   st->dt = dt;
   st->sim_time = t;
-}
-__global__ void k_accumulate_totals(DevState *st, long long *tot, int grid) {
-  tot[0] += 1;                  // steps
-  tot[1] += st->iterations;     // Jacobi iterations
-  tot[2] += st->converged ? 0 : 1;
-  tot[3] += st->ap_evals;
-  tot[4] |= st->status;
-  tot[5] = tot[5] > (long long)st->bar_seen ? tot[5] : (long long)st->bar_seen;
-  tot[6] = grid;
 }
 
 static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -835,7 +825,6 @@ int wsph_pressure_solve(wsph_solver *s, wsph_step_stats *stats) {
     // A p0 and the first update (IISPH.py:1155-1198, 1209-1219)
     launch_j1(b, s->C, 0, P.block, s->stream);
     launch_j2(b, s->C, 0, -1, P.block, s->stream);
-    k_add_ap_evals<<<1, 1, 0, s->stream>>>(s->st, 1);
     src = 1;
   }
   if (P.jacobi_mode == WSPH_JACOBI_PERSISTENT) {
@@ -921,10 +910,15 @@ int wsph_integrate(wsph_solver *s, wsph_step_stats *stats) {
                            s->stream));
   }
   // final_dp -1: index kept on the device
-  launch_integrate(b, s->C, s->final_dp, s->P.warm_start, s->P.block, s->stream);
+  StepEnd end;
+  end.dt_mode = s->P.dt_mode;
+  end.cfl_d = s->P.cfl * s->P.spacing;
+  end.inv_small = s->P.inv_small;
+  end.dt_max = s->P.dt_max;
+  end.totals = s->totals;
+  end.grid = s->persistent_grid;
+  launch_integrate(b, s->C, s->final_dp, s->P.warm_start, s->P.block, end, s->stream);
   s->have_pairs = s->log_pairs && s->n > 0;
-  launch_clock(b, s->P, s->stream);
-  k_accumulate_totals<<<1, 1, 0, s->stream>>>(s->st, s->totals, s->persistent_grid);
   rec(s, 4);
   HIP_TRY(hipGetLastError());
   s->searched = s->advected = s->solved = false;
@@ -974,7 +968,6 @@ static int enqueue_front(wsph_solver *s, int c) {
   if (P.warm_start != WSPH_WARM_ZERO) {
     launch_j1(b, s->C, 0, P.block, s->stream);
     launch_j2(b, s->C, 0, -1, P.block, s->stream);
-    k_add_ap_evals<<<1, 1, 0, s->stream>>>(s->st, 1);
     src = 1;
   }
   s->graph_src = src;

@@ -18,17 +18,20 @@ __global__ void k_fill_u32(unsigned *p, unsigned v, long n) {
 
 // Cell slot of every particle plus its arrival rank in that cell.
 // With scenes, the point's own scene grid is used and its slot offset by the
-// scene's base, so the scenes' cells never mix.
+// scene's base, so the scenes' cells never mix. The particles arrive in last
+// step's cell order, so a wave holds a few runs of equal slots: one atomic per
+// distinct slot and wave hands out the ranks (their order inside a cell does
+// not matter: k_order_in_cell orders each cell by original index).
 __global__ void k_cell_slots(const float4 *__restrict__ pos, int n, float inv, int d0, int d1,
                              int d2, int layout, int *__restrict__ slot, int *__restrict__ rank,
                              unsigned *__restrict__ count, const int *__restrict__ sid,
                              const SceneGrid *__restrict__ scenes, int boundary) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 p = pos[i];
+  const bool live = i < n;
+  const float4 p = live ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   int dims[3] = {d0, d1, d2};
   int base = 0;
-  if (scenes) {
+  if (scenes && live) {
     const SceneGrid &g = scenes[sid[i]];
     const int *d = boundary ? g.bdims : g.fdims;
     dims[0] = d[0];
@@ -39,9 +42,24 @@ __global__ void k_cell_slots(const float4 *__restrict__ pos, int n, float inv, i
   const int hx = hash_axis((int)(p.x * inv), dims[0]);
   const int hy = hash_axis((int)(p.y * inv), dims[1]);
   const int hz = hash_axis((int)(p.z * inv), dims[2]);
-  const int c = base + cell_slot(hx, hy, hz, dims, layout);
-  slot[i] = c;
-  rank[i] = (int)atomicAdd(&count[c], 1u);
+  const int c = live ? base + cell_slot(hx, hy, hz, dims, layout) : -1;
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(live);
+  int r = 0;
+  while (pending) {  // wave-uniform: one pass per distinct slot of the wave
+    const int leader = __ffsll((long long)pending) - 1;
+    const int cl = __shfl(c, leader, 64);
+    const unsigned long long same = __ballot(c == cl);
+    unsigned base_rank = 0;
+    if (lane == leader) base_rank = atomicAdd(&count[cl], (unsigned)__popcll(same));
+    base_rank = __shfl(base_rank, leader, 64);
+    if (c == cl) r = (int)base_rank + __popcll(same & ((1ull << lane) - 1ull));
+    pending &= ~same;
+  }
+  if (live) {
+    slot[i] = c;
+    rank[i] = r;
+  }
 }
 
 __global__ void k_place(const int *__restrict__ slot, const int *__restrict__ rank,

@@ -99,7 +99,8 @@ void launch_assemble(const Bufs &b, const SphConst &c, int start_mode, int block
 struct Weights;
 void launch_learned_start(const Bufs &b, const SphConst &c, const Weights *net, hipStream_t s);
 // One A p evaluation on dp[src]; writes dp[1-src] = next iterate. slot >= 0
-// accumulates the residual of iteration `slot`.
+// accumulates the residual of iteration `slot`; slot == -1 marks the
+// evaluation of A p0 (counted in DevState::ap_evals).
 void launch_j1(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s);
 void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, hipStream_t s);
 // `iters` Jacobi iterations with device-side stop test (no-ops once done);
@@ -107,10 +108,17 @@ void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, h
 // the chunk's first iteration
 void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P, int src0,
                          int iters, int block, hipStream_t s, hipEvent_t *ev3 = nullptr);
+// Step-end bookkeeping done by the statistics kernel after the kick/drift:
+// clock and CFL dt update (IISPH.py:1367-1371) and the cumulative counters.
+struct StepEnd {
+  int dt_mode;
+  double cfl_d, inv_small, dt_max;
+  long long *totals;  // cumulative counters (wsph_get_counters)
+  int grid;           // persistent grid of the step (0: none)
+};
 // src < 0: read the final buffer index from DevState
 void launch_integrate(const Bufs &b, const SphConst &c, int src, int start_mode, int block,
-                      hipStream_t s);
-void launch_clock(const Bufs &b, const wsph_params &P, hipStream_t s);
+                      const StepEnd &end, hipStream_t s);
 void launch_reset_state(const Bufs &b, hipStream_t s);
 // Batched scenes: after the Jacobi loop, move every scene's final iterate
 // into dp[final_src] (scenes that stopped early hold it in the other buffer).

@@ -859,6 +859,7 @@ __global__ void k_j2(Bufs b, SphConst c, int src, int slot) {
   unsigned rc = 0;
   if (i < b.n) j2_particle(b, c, T, i, src, rs, rc);
   if (slot >= 0) block_residual_store(b, rs, rc);
+  if (slot == -1 && blockIdx.x == 0 && threadIdx.x == 0) b.st->ap_evals += 1;  // the A p0 evaluation
 }
 
 // ---- device-resident Jacobi loop control ----------------------------------
@@ -1012,7 +1013,7 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
 // max |v| and the step statistics from the per-block partials of
 // k_integrate (summed in block order: deterministic), appended to the
 // history ring.
-__global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode) {
+__global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode, StepEnd end) {
   __shared__ unsigned s[16], sy[16];
   __shared__ double sk[16], ss[16], sn[16], sc[16];
   unsigned m = 0, my = 0;
@@ -1086,18 +1087,24 @@ __global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode) {
       }
     }
     st->step_count += 1;
-  }
-}
-
-// sim_time += dt; CFL dt update (IISPH.py:1367-1371)
-__global__ void k_clock(DevState *st, int dt_mode, double cfl_d, double inv_small, double dt_max) {
-  const double dt = st->dt;
-  st->sim_time = st->sim_time + dt;
-  if (dt_mode == WSPH_DT_CFL) {
-    const double vmax = (double)__uint_as_float(st->vmax_bits);
-    const double m = vmax > inv_small ? vmax : inv_small;
-    const double nd = cfl_d / m;
-    st->dt = nd < dt_max ? nd : dt_max;
+    // sim_time += dt; CFL dt update (IISPH.py:1367-1371)
+    const double dt = st->dt;
+    st->sim_time = st->sim_time + dt;
+    if (end.dt_mode == WSPH_DT_CFL) {
+      const double vmax = (double)__uint_as_float(st->vmax_bits);
+      const double mv = vmax > end.inv_small ? vmax : end.inv_small;
+      const double nd = end.cfl_d / mv;
+      st->dt = nd < end.dt_max ? nd : end.dt_max;
+    }
+    // cumulative counters
+    long long *tot = end.totals;
+    tot[0] += 1;
+    tot[1] += st->iterations;
+    tot[2] += st->converged ? 0 : 1;
+    tot[3] += st->ap_evals;
+    tot[4] |= st->status;
+    tot[5] = tot[5] > (long long)st->bar_seen ? tot[5] : (long long)st->bar_seen;
+    tot[6] = end.grid;
   }
 }
 
@@ -1348,14 +1355,9 @@ void launch_batch_finalize(const Bufs &b, hipStream_t s) {
 }
 
 void launch_integrate(const Bufs &b, const SphConst &c, int src, int start_mode, int block,
-                      hipStream_t s) {
-  if (b.n == 0) return;
-  k_integrate<<<nblocks(b.n, block), block, 0, s>>>(b, c, src);
-  k_vmax_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block), start_mode);
-}
-
-void launch_clock(const Bufs &b, const wsph_params &P, hipStream_t s) {
-  k_clock<<<1, 1, 0, s>>>(b.st, P.dt_mode, P.cfl * P.spacing, P.inv_small, P.dt_max);
+                      const StepEnd &end, hipStream_t s) {
+  if (b.n > 0) k_integrate<<<nblocks(b.n, block), block, 0, s>>>(b, c, src);
+  k_vmax_reduce<<<1, 1024, 0, s>>>(b, b.n > 0 ? (int)nblocks(b.n, block) : 0, start_mode, end);
 }
 
 void launch_reset_state(const Bufs &b, hipStream_t s) {
