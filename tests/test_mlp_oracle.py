@@ -93,3 +93,47 @@ def test_adam_matches_torch():
         opt.step()
         p, m, v = M.adam(p, m, v, g, step, 1e-3)
         np.testing.assert_allclose(p, P.detach().numpy(), rtol=0, atol=1e-12)
+
+
+def test_feature_scale_matches_lattice_closed_form(orc):
+    """a_ref of the learned start's features (oracle/learn_oracle.py) equals
+    the closed form of a_ii for a particle inside a resting lattice at rest
+    density, -(dt/d)^2 * 0.42483995 (the same lattice sum pinned in
+    test_oracle), and the IISPH oracle's interior a_ii divided by it is
+    (rho0/rho)^2."""
+    import math
+    import learn_oracle as LO
+    from helpers import oracle_setup
+    from warpsph_amd import config
+    for cfg in (config.dambreak_8k(), config.dambreak_1m(), config.torus_4m()):
+        want = -(cfg.dt0 / cfg.spacing) ** 2 * 0.42483995
+        assert abs(LO.a_ref(cfg, cfg.dt0) / want - 1) < 1e-7
+    cfg = config.dambreak_8k().replace(fluid_cells=(12, 12, 12), xsph=0.0, force_iters=2)
+    sc, P, wt, gt = oracle_setup(cfg)
+    psi = orc.boundary_psi(P, wt, sc.xb)
+    _, _, info = orc.step(P, wt, gt, sc.x, np.zeros_like(sc.x), sc.xb, sc.vb, psi)
+    k = (6 * 12 + 6) * 12 + 6
+    f1 = info["a_ii"][k] / LO.a_ref(cfg, cfg.dt0)
+    assert abs(f1 - (cfg.rho0 / info["rho"][k]) ** 2) < 1e-4
+    # the device's constant: a_ii * k1 / dt^2 with k1 = -rho0^2 / (m^2 S)
+    S = LO.lattice_grad_sq_sum(cfg.h, cfg.spacing)
+    k1 = -(cfg.rho0 ** 2) / (cfg.mass ** 2 * S)
+    assert abs(k1 * LO.a_ref(cfg, cfg.dt0) / cfg.dt0 ** 2 - 1) < 1e-12
+    assert math.isfinite(k1)
+
+
+def test_start_value_rule():
+    import learn_oracle as LO
+    from warpsph_amd import config
+    cfg = config.dambreak_8k()
+    carry = np.array([0.0, 100.0, 100.0, 100.0, 50.0], np.float32)
+    y = np.array([0.5, -1.0, np.nan, 0.2, 1.0], np.float32)
+    a = np.array([-1e-3, -1e-3, -1e-3, -1e-3, 5e-7], np.float32)
+    f1 = np.array([1.0, 1.0, 1.0, 0.01, 1.0], np.float32)
+    p = LO.start_value(carry, y, a, cfg, f1)
+    P = np.float32(LO.p_ref(cfg))
+    assert p[0] == np.float32(0.0) + P * np.float32(0.5)
+    assert p[1] == 0.0                      # clamped at 0
+    assert p[2] == 100.0                    # non-finite prediction: carried value
+    assert p[3] == 100.0                    # weakly coupled: carried value
+    assert p[4] == 0.0                      # guard band |a_ii| < inv_small
