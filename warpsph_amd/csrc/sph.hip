@@ -240,6 +240,40 @@ __device__ __forceinline__ void for_query_rows(float px, float py, float pz, boo
   }
 }
 
+// Candidates [s0, e0) of one span tested against p without branches, 32 at a
+// time, into a per-lane hit mask; `put` then runs once per hit, in ascending
+// candidate order (the order of a one-by-one scan). self: the index to skip
+// (the particle itself), -1 for none.
+template <typename PutF>
+__device__ __forceinline__ void span_hits(const float4 *__restrict__ pts, int s0, int e0,
+                                          const float4 &p, float r2_lt, int self, PutF put) {
+  for (int base = s0; base < e0; base += 32) {
+    const int cnt = e0 - base < 32 ? e0 - base : 32;
+    unsigned m = 0u;
+    int u = 0;
+    for (; u + kBatch <= cnt; u += kBatch) {
+      float4 o[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) o[k] = pts[base + u + k];
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) {
+        const bool hit = (base + u + k != self) & (dist2(p.x, p.y, p.z, o[k].x, o[k].y, o[k].z) < r2_lt);
+        m |= (unsigned)hit << (u + k);
+      }
+    }
+    for (; u < cnt; ++u) {
+      const float4 o = pts[base + u];
+      const bool hit = (base + u != self) & (dist2(p.x, p.y, p.z, o.x, o.y, o.z) < r2_lt);
+      m |= (unsigned)hit << u;
+    }
+    while (m) {
+      const int bit = __builtin_ctz(m);
+      m &= m - 1u;
+      put(base + bit);
+    }
+  }
+}
+
 // Neighbour list writer of one wave. Entry k of lane l belongs at
 // list[k * n + i_l]; the lanes find their entries at different times, and
 // storing each as found fills a 128-byte line of slot k from ~32 separate
@@ -297,10 +331,8 @@ __global__ void k_lists(Bufs b, SphConst c) {
   for_query_rows(
       p.x, p.y, p.z, live, c, fg.dims, fg.start, fg.count,
       [&](int s0, int e0) {
-        for_span(b.pos, s0, e0, [&](int j, const float4 &o) {
-          // distance < SMOOTHING_LENGTH and index != i (IISPH.py:72)
-          if (j != i && dist2(p.x, p.y, p.z, o.x, o.y, o.z) < c.r2_lt) wf.put(j);
-        });
+        // distance < SMOOTHING_LENGTH and index != i (IISPH.py:72)
+        span_hits(b.pos, s0, e0, p, c.r2_lt, i, [&](int j) { wf.put(j); });
       },
       [&]() { wf.row_done(); });
   wf.finish();
@@ -310,10 +342,8 @@ __global__ void k_lists(Bufs b, SphConst c) {
     for_query_rows(
         p.x, p.y, p.z, live, c, bg.dims, bg.start, bg.count,
         [&](int s0, int e0) {
-          for_span(b.bpos, s0, e0, [&](int j, const float4 &o) {
-            // distance < SMOOTHING_LENGTH (IISPH.py:129)
-            if (dist2(p.x, p.y, p.z, o.x, o.y, o.z) < c.r2_lt) wb.put(j);
-          });
+          // distance < SMOOTHING_LENGTH (IISPH.py:129)
+          span_hits(b.bpos, s0, e0, p, c.r2_lt, -1, [&](int j) { wb.put(j); });
         },
         [&]() { wb.row_done(); });
     wb.finish();
