@@ -100,12 +100,18 @@ def traffic_from_profile(workload: str):
         return None
 
 
-def cpu_baseline(cfg, x, v, budget_s: float):
+def _nproc() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(cfg, x, v, budget_s: float, threads: int = 1):
     sys.path.insert(0, str(ROOT / "oracle"))
-    import numpy as np
     import oracle as O
     from warpsph_amd import scene as scn
-    O.set_threads(1)
+    O.set_threads(threads)
     sc = scn.build(cfg)
     sc.x = x
     O_s = O.OracleSolver(cfg, sc)
@@ -119,10 +125,11 @@ def cpu_baseline(cfg, x, v, budget_s: float):
         if el >= budget_s * 0.5 or el * (steps + 1) / steps > budget_s:
             break
     el = time.perf_counter() - t0
-    return {"value": cfg_n(x) * steps / el, "unit": "fluid particle-steps/s", "cores": 1,
+    return {"value": cfg_n(x) * steps / el, "unit": "fluid particle-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"{cfg_n(x)}-particle state at the end of the GPU run, {steps} step(s), "
-                      f"{el:.1f} s, one thread, float32 oracle restating the reference step"}
+                      f"{el:.1f} s, {threads} thread(s) (OpenMP), float32 oracle restating the "
+                      f"reference step with domain-covering hash dims"}
 
 
 def cfg_n(x):
@@ -382,9 +389,15 @@ def main():
         if "previous" in plain:
             line["speedup_vs_previous_start"] = r["value"] / plain["previous"]["value"]
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(base.replace(warm_start=config.WARM_PREVIOUS), x_end,
-                                            v_end, args.cpu_seconds)
+        ccfg = base.replace(warm_start=config.WARM_PREVIOUS)
+        line["cpu_baseline"] = cpu_baseline(ccfg, x_end, v_end, args.cpu_seconds)
         line["cpu_baseline"]["host_cpu"] = _cpu_model()
+        line["cpu_baseline"]["nproc"] = _nproc()
+        # the same oracle on every core the process may use
+        par = cpu_baseline(ccfg, x_end, v_end, args.cpu_seconds / 2, threads=_nproc())
+        par["host_cpu"] = _cpu_model()
+        par["nproc"] = _nproc()
+        line["cpu_baseline_all_cores"] = par
     print(json.dumps(line), flush=True)
 
 
