@@ -64,6 +64,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--layout", type=int, default=-1, help="cell layout: 0 linear, 1 Morton")
+    ap.add_argument("--block", type=int, default=0, help="threads per block (0: solver default)")
     ap.add_argument("--coop-per-cu", type=int, default=0,
                     help="persistent kernel blocks per CU (0: solver default)")
     ap.add_argument("--trace-steps", type=int, default=0,
@@ -188,20 +189,24 @@ def bench_batch(args) -> None:
     import torch
     from warpsph_amd import autotune, config, scene
     from warpsph_amd.batch import BatchIISPH
-    start = {"zero": config.WARM_ZERO, "previous": config.WARM_PREVIOUS}.get(args.warm_start)
-    if start is None:
-        raise SystemExit("the batch workload runs the plain starts (--warm-start previous|zero)")
+    start = {"zero": config.WARM_ZERO, "previous": config.WARM_PREVIOUS,
+             "learned": config.WARM_LEARNED}[args.warm_start]
+    net = None
+    if start == config.WARM_LEARNED:
+        from warpsph_amd.mlp import PressureNet
+        net = PressureNet.load(args.net, max_batch=1024)
     cfgs = [c.replace(warm_start=start) for c in scene.batch_configs(config.preset(args.preset))]
     scs = [scene.build(c) for c in cfgs]
 
     def make(shape_cfg):
         return BatchIISPH([c.replace(**{k: getattr(shape_cfg, k) for k in autotune.SHAPE_KEYS})
-                           for c in cfgs], scs)
+                           for c in cfgs], scs, predictor=net)
 
     tuned = None
     shape_cfg = cfgs[0]
     if args.autotune != "off":
         tuned = autotune.tune(make, cfgs[0], key=f"{args.preset}|start{start}|n{sum(x.n for x in scs)}",
+                              warmup=20,
                               force=args.autotune == "force",
                               log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
         shape_cfg = autotune.apply(cfgs[0], tuned["shape"])
@@ -277,8 +282,6 @@ def main():
     WARM.update(zero=config.WARM_ZERO, previous=config.WARM_PREVIOUS,
                 learned=config.WARM_LEARNED)
     if config.preset(args.preset).scene == "batch":
-        if args.warm_start == "learned":
-            args.warm_start = "previous"
         bench_batch(args)
         return
 
@@ -289,6 +292,8 @@ def main():
                         chunk_iters=args.chunk)
     if args.layout >= 0:
         base = base.replace(cell_layout=args.layout)
+    if args.block:
+        base = base.replace(block=args.block)
     if args.coop_per_cu:
         base = base.replace(coop_blocks_per_cu=args.coop_per_cu)
     net = None
@@ -310,7 +315,7 @@ def main():
         from warpsph_amd import scene as scn
         sc = scn.build(tune_cfg)
         tuned = autotune.tune(lambda c: IISPH(stage_path=None, config=c, scene=sc, predictor=net),
-                              tune_cfg,
+                              tune_cfg, warmup=20,
                               key=key, cands=cands, force=args.autotune == "force",
                               log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
         base = autotune.apply(base, tuned["shape"])
