@@ -1,0 +1,111 @@
+"""Edge cases of the HIP path: no fluid, isolated particles, no boundary,
+neighbour-list capacity overflow (an error, where the reference overruns its
+n*60 slots), a batch containing an empty scene."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import assert_same
+from warpsph_amd import config, scene
+from warpsph_amd import _native as N
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _solver(cfg, sc):
+    from warpsph_amd.solver import IISPH
+    return IISPH(stage_path=None, config=cfg, scene=sc, collect_stats=True)
+
+
+def _scene(x, xb):
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+    xb = np.ascontiguousarray(xb, np.float32).reshape(-1, 3)
+    pts = [a for a in (x, xb) if a.size]
+    lo = np.min([a.min(0) for a in pts], 0).astype(np.float64)
+    hi = np.max([a.max(0) for a in pts], 0).astype(np.float64)
+    return scene.Scene(x, xb, np.zeros_like(xb), xb.shape[0], lo, hi)
+
+
+def test_no_fluid():
+    _announce("edge: no fluid particles")
+    cfg = config.dambreak_8k()
+    base = scene.build(cfg)
+    s = _solver(cfg, _scene(np.zeros((0, 3)), base.xb))
+    for _ in range(3):
+        s.step()
+    st = s.step_stats()
+    assert st["iterations"] == 0 and st["converged"]
+    assert s.x.shape == (0, 3)
+    assert s.counters()["steps"] == 3
+
+
+@pytest.mark.parametrize("start", [config.WARM_ZERO, config.WARM_PREVIOUS])
+def test_isolated_particles_match_oracle(orc, start):
+    _announce(f"edge: isolated particles, start {start}")
+    cfg = config.dambreak_8k().replace(warm_start=start)
+    base = scene.build(cfg)
+    # a few particles far from each other and from the walls, one pair within h
+    x = np.array([[0.3, 0.5, 0.3], [0.6, 0.6, 0.6], [0.5, 0.2, 0.7], [0.51, 0.2, 0.7]])
+    sc = _scene(x, base.xb)
+    s = _solver(cfg, sc)
+    o = orc.OracleSolver(cfg, sc)
+    for _ in range(5):
+        s.step()
+        info = o.step()
+    assert_same("x", s.x.cpu().numpy(), o.x)
+    assert_same("v", s.v.cpu().numpy(), o.v)
+    assert_same("p", s.p.cpu().numpy(), info["p"])
+    assert np.all(np.isfinite(s.x.cpu().numpy()))
+
+
+def test_no_boundary_free_fall_matches_oracle(orc):
+    _announce("edge: no boundary particles")
+    cfg = config.dambreak_8k().replace(fluid_cells=(6, 6, 6), warm_start=config.WARM_PREVIOUS)
+    x = scene.lattice_block(cfg.fluid_cells, cfg.spacing, (0.3, 0.3, 0.3))
+    sc = _scene(x, np.zeros((0, 3)))
+    s = _solver(cfg, sc)
+    o = orc.OracleSolver(cfg, sc)
+    for _ in range(4):
+        s.step()
+        info = o.step()
+    assert_same("x", s.x.cpu().numpy(), o.x)
+    assert_same("p", s.p.cpu().numpy(), info["p"])
+    assert s.fs_neighbor_num.cpu().numpy().max() == 0
+
+
+def test_neighbour_capacity_overflow_is_an_error():
+    _announce("edge: neighbour capacity overflow")
+    cfg = config.dambreak_8k().replace(max_ff=8)
+    s = _solver(cfg, scene.build(cfg))
+    with pytest.raises(RuntimeError, match="capacity"):
+        s.step()
+    # a solver with room continues normally on the same scene
+    cfg2 = config.dambreak_8k()
+    s2 = _solver(cfg2, scene.build(cfg2))
+    s2.step()
+    assert s2.step_stats()["max_ff_count"] <= cfg2.max_ff
+
+
+def test_batch_with_an_empty_scene():
+    from warpsph_amd.batch import BatchIISPH
+    from warpsph_amd.solver import IISPH
+    _announce("edge: batch with an empty scene")
+    cfgs = [c.replace(warm_start=config.WARM_PREVIOUS)
+            for c in scene.batch_configs(config.batch_256())[:3]]
+    scs = [scene.build(c) for c in cfgs]
+    scs[1] = _scene(np.zeros((0, 3)), scs[1].xb)
+    b = BatchIISPH(cfgs, scs)
+    for _ in range(4):
+        b.step()
+    st = b.scene_stats()
+    assert st["converged"].all()
+    s0 = IISPH(stage_path=None, config=cfgs[0], scene=scs[0])
+    for _ in range(4):
+        s0.step()
+    assert_same("scene 0 x", b.split(b.x.cpu().numpy())[0], s0.x.cpu().numpy())

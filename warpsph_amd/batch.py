@@ -129,6 +129,8 @@ class BatchIISPH:
         dtype = torch.int32 if fid in N.FIELD_IS_INT else torch.float32
         out = torch.empty((count, width) if width > 1 else (count,), dtype=dtype,
                           device=self.device)
+        if count == 0:
+            return out
         torch.cuda.synchronize()
         N.check("wsph_get_field", self._lib.wsph_get_field(self._h, fid, out.data_ptr()))
         return out

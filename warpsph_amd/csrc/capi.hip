@@ -629,7 +629,7 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
     for (int v : s->scene_nf) total += v;
     if (total != n) return fail(WSPH_E_ARG, "fluid count %d differs from the scenes' %ld", n, total);
   }
-  if (n != s->cap_n) {
+  if (n != s->cap_n || !s->pos[0]) {  // (a first call with n = 0 allocates too)
     free_fluid(s);
     for (int k = 0; k < 2; ++k) {
       HIP_TRY(dalloc(&s->pos[k], n));
@@ -820,6 +820,13 @@ int wsph_pressure_solve(wsph_solver *s, wsph_step_stats *stats) {
   rec(s, 2);
   const Bufs b = make_bufs(s);
   const wsph_params &P = s->P;
+  if (s->n == 0) {  // nothing to solve: a converged step with no iterations
+    k_solve_result<<<1, 1, 0, s->stream>>>(s->st, 0, 0.0, 1, 0);
+    HIP_TRY(hipGetLastError());
+    s->final_dp = 0;
+    s->solved = true;
+    return WSPH_OK;
+  }
   int src = 0;
   if (P.warm_start != WSPH_WARM_ZERO) {
     // A p0 and the first update (IISPH.py:1155-1198, 1209-1219)

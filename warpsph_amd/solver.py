@@ -206,6 +206,8 @@ class IISPH:
         dtype = torch.int32 if fid in N.FIELD_IS_INT else torch.float32
         shape = (count, width) if width > 1 else (count,)
         out = torch.empty(shape, dtype=dtype, device=self.device)
+        if count == 0:
+            return out
         _sync_torch()
         N.check("wsph_get_field", self._lib.wsph_get_field(self._h, fid, out.data_ptr()))
         return out
@@ -217,6 +219,8 @@ class IISPH:
         cap = cap or (self.cfg.max_ff if which == 0 else self.cfg.max_fs)
         cnt = torch.zeros(self.n, dtype=torch.int32, device=self.device)
         idx = torch.full((self.n, cap), -1, dtype=torch.int32, device=self.device)
+        if self.n == 0:
+            return cnt, idx
         _sync_torch()
         N.check("wsph_get_neighbors", self._lib.wsph_get_neighbors(
             self._h, which, cnt.data_ptr(), idx.data_ptr(), cap))
