@@ -283,7 +283,8 @@ __device__ __forceinline__ void span_hits(const float4 *__restrict__ pts, int s0
 // coalesced row, when every lane of the wave has passed it (after each query
 // row) or at the end. A lane more than kListRing slots ahead of the wave's
 // flushed rows stores its later entries directly.
-constexpr int kListRing = 16;
+constexpr int kListRing = 16;  // a power of two
+static_assert((kListRing & (kListRing - 1)) == 0, "ring rows: a power of two");
 struct ListWriter {
   int *ring;      // [kListRing][64] of this wave
   int *list;
@@ -294,7 +295,7 @@ struct ListWriter {
   __device__ void put(int j) {
     if (cnt < cap) {
       if (direct == 0x7fffffff && cnt - flushed < kListRing) {
-        ring[(cnt % kListRing) * 64 + lane] = j;
+        ring[((unsigned)cnt & (kListRing - 1)) * 64 + lane] = j;
       } else {
         if (direct == 0x7fffffff) direct = cnt;
         list[(long)cnt * n + i] = j;
@@ -304,7 +305,7 @@ struct ListWriter {
   }
   __device__ void flush_to(int upto) {  // wave-uniform
     for (int k = flushed; k < upto; ++k)
-      if (live && k < cnt && k < cap && k < direct) list[(long)k * n + i] = ring[(k % kListRing) * 64 + lane];
+      if (live && k < cnt && k < cap && k < direct) list[(long)k * n + i] = ring[((unsigned)k & (kListRing - 1)) * 64 + lane];
     flushed = upto > flushed ? upto : flushed;
   }
   __device__ void row_done() {  // wave-uniform: rows every lane has completed
