@@ -84,6 +84,8 @@ enum {
   WSPH_F_FS_COUNT = 11, /* int[n]    boundary neighbour count IISPH.py:802 */
   WSPH_F_PSI = 12,    /* float[nb]   boundary psi           IISPH.py:789 */
   WSPH_F_CELL_KEY = 13, /* int[n]    linear hash key of each particle */
+  WSPH_F_BX = 14,     /* float[nb][3] boundary positions    IISPH.py:989 */
+  WSPH_F_BV = 15,     /* float[nb][3] boundary velocities   IISPH.py:991 */
   WSPH_F_COUNT
 };
 
@@ -182,6 +184,19 @@ int wsph_set_tables(wsph_solver *s, const float *w_table, int w_size,
  * zero velocity), build the boundary hash grid and compute psi
  * (IISPH.py:811-821 -> compute_boundary_density, IISPH.py:23-47). */
 int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb);
+
+/* Rigid motion of the boundary particles with original index >= first (the
+ * moving model that follows the walls; the reference's boundary_hide_n,
+ * IISPH.py:927). The step that starts at sim_time 0 sets their velocity to
+ * v[0..2] (set_rigid_v, IISPH.py:652-660, launched at IISPH.py:1061-1081,
+ * before predict_rho_adv); every step moves them by velocity * dt after the
+ * fluid drift (update_rigid_x, IISPH.py:663-671, launched at
+ * IISPH.py:1355-1364); and every neighbour search rebuilds the boundary hash
+ * grid from the moved positions (DYNAMIC_SCENE, IISPH.py:1415-1416). psi
+ * keeps its set-up values, as in the reference. Call after
+ * wsph_set_boundary, which clears it; first < 0 clears it; not available
+ * with batched scenes (WSPH_E_UNSUPPORTED). */
+int wsph_set_rigid(wsph_solver *s, int first, const float *v);
 
 /* Upload fluid state (device float[n][3]; v may be NULL for rest), replaces
  * the wp.array copies at IISPH.py:990-994. */

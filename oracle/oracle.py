@@ -152,9 +152,19 @@ def step(params, w_table, g_table, x, v, xb, vb, psi, p0=None, residual_mode: in
 
 class OracleSolver:
     """Multi-step driver over ``step`` that keeps the host clock like
-    IISPH.step does (sim_time += dt; CFL dt update, IISPH.py:1367-1371)."""
+    IISPH.step does (sim_time += dt; CFL dt update, IISPH.py:1367-1371).
 
-    def __init__(self, cfg, scene, grid_dims=None):
+    With ``cfg.rigid_v`` set, the boundary particles from
+    ``scene.boundary_hide_n`` on form a moving model: in the step at
+    sim_time 0 their velocity becomes rigid_v (set_rigid_v, IISPH.py:652-660,
+    launched at :1061-1081, before predict_rho_adv, the first reader of the
+    boundary velocity), after each step they advance by v * dt in float32
+    (update_rigid_x, IISPH.py:663-671, launched at :1355-1364), and the
+    next step's boundary grid is built from the moved positions (``step``
+    builds it from xb on every call: DYNAMIC_SCENE, IISPH.py:1415-1416). psi
+    keeps its initial values."""
+
+    def __init__(self, cfg, scene, grid_dims=None, rigid_first=None):
         from warpsph_amd import scene as scn
         from warpsph_amd import tables
         from warpsph_amd._native import make_params
@@ -164,8 +174,13 @@ class OracleSolver:
         self.wt, self.gt = tables.make_tables(cfg.h, cfg.w_size, cfg.grad_size)
         self.x = scene.x.copy()
         self.v = np.zeros_like(self.x)
-        self.xb, self.vb = scene.xb, scene.vb
+        self.xb = _f32(scene.xb, (-1, 3)).copy()
+        self.vb = _f32(scene.vb, (-1, 3)).copy()
         self.psi = boundary_psi(self.params, self.wt, self.xb)
+        rv = getattr(cfg, "rigid_v", None)
+        self.rigid_first = (None if rv is None else scene.boundary_hide_n) \
+            if rigid_first is None else rigid_first
+        self.rigid_v = np.asarray(rv if rv is not None else (0.0, 0.0, 0.0), dtype=np.float32)
         self.p = np.zeros(self.x.shape[0], dtype=np.float32)
         self.p_carry = self.p.copy()
         self.dt = cfg.dt0
@@ -177,8 +192,13 @@ class OracleSolver:
         self.params.dt = self.dt
         if p0 is None and cfg.warm_start == 1:
             p0 = self.p_carry
+        k = self.rigid_first
+        if k is not None and self.sim_time == 0.0:
+            self.vb[k:] = self.rigid_v
         self.x, self.v, info = step(self.params, self.wt, self.gt, self.x, self.v, self.xb,
                                     self.vb, self.psi, p0=p0, want_fields=True)
+        if k is not None:
+            self.xb[k:] = self.xb[k:] + self.vb[k:] * np.float32(self.dt)
         self.p = info["p"]
         self.p_carry = info["p_carry"]
         info["dt"] = self.dt

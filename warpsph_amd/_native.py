@@ -17,10 +17,12 @@ LIB_PATH = PKG_DIR / "libwsph.so"
 
 WSPH_OK = 0
 F_X, F_V, F_P, F_RHO, F_RHO_ADV, F_V_ADV, F_D_II, F_A_II, F_AP, F_S, \
-    F_FF_COUNT, F_FS_COUNT, F_PSI, F_CELL_KEY = range(14)
+    F_FF_COUNT, F_FS_COUNT, F_PSI, F_CELL_KEY, F_BX, F_BV = range(16)
 FIELD_WIDTH = {F_X: 3, F_V: 3, F_P: 1, F_RHO: 1, F_RHO_ADV: 1, F_V_ADV: 3,
                F_D_II: 3, F_A_II: 1, F_AP: 1, F_S: 3, F_FF_COUNT: 1,
-               F_FS_COUNT: 1, F_PSI: 1, F_CELL_KEY: 1}
+               F_FS_COUNT: 1, F_PSI: 1, F_CELL_KEY: 1, F_BX: 3, F_BV: 3}
+#: fields with one row per boundary particle
+BOUNDARY_FIELDS = {F_PSI, F_BX, F_BV}
 FIELD_IS_INT = {F_FF_COUNT, F_FS_COUNT, F_CELL_KEY}
 
 
@@ -108,6 +110,7 @@ SIGNATURES = [
     ("wsph_set_tables", _I, [_P, _P, _I, _P, _I]),
     ("wsph_set_boundary", _I, [_P, _P, _P, _I]),
     ("wsph_set_fluid", _I, [_P, _P, _P, _I]),
+    ("wsph_set_rigid", _I, [_P, _I, _P]),
     ("wsph_set_pressure", _I, [_P, _P]),
     ("wsph_step", _I, [_P, C.POINTER(WsphStepStats)]),
     ("wsph_neighbor_search", _I, [_P]),

@@ -41,6 +41,8 @@ class BatchIISPH:
         if not cfgs:
             raise ValueError("empty batch")
         base = cfgs[0]
+        if any(c.rigid_v is not None for c in cfgs):
+            raise ValueError("rigid boundary motion is not available with batched scenes")
         for k, c in enumerate(cfgs):
             for f in _SHARED:
                 if getattr(c, f) != getattr(base, f):
@@ -125,7 +127,7 @@ class BatchIISPH:
         """A field of all scenes (concatenated scene order)."""
         torch = _torch()
         width = N.FIELD_WIDTH[fid]
-        count = self.boundary_n if fid == N.F_PSI else self.n
+        count = self.boundary_n if fid in N.BOUNDARY_FIELDS else self.n
         dtype = torch.int32 if fid in N.FIELD_IS_INT else torch.float32
         out = torch.empty((count, width) if width > 1 else (count,), dtype=dtype,
                           device=self.device)

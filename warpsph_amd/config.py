@@ -72,6 +72,9 @@ class SolverConfig:
     coop_blocks_per_cu: int = 0
     chunk_iters: int = 0
     graph_iters: int = 0           # >0: replay the step as a captured hipGraph
+    # velocity of the moving model (boundary particles after the walls); None: static
+    # boundary (SceneType.PLANE / HAND set (2, 0, 0) / (0, 0.1, 0), IISPH.py:1061-1081)
+    rigid_v: Optional[Tuple[float, float, float]] = None
     extra: dict = field(default_factory=dict)
 
     # ---- derived quantities (wp_sph/main.py:57-60) ---------------------
@@ -162,6 +165,25 @@ def learned_1m() -> SolverConfig:
     return dambreak_1m().replace(warm_start=WARM_LEARNED)
 
 
+def paddle_8k() -> SolverConfig:
+    """dambreak_8k with a paddle of 3 x 10 x 10 boundary particles at
+    (0.44, 0, 0.1), just beyond the fluid block's +x face, pushed into the
+    fluid at 1 m/s along -x (a moving model, as the reference's PLANE and
+    HAND scenes); 200 steps."""
+    return dambreak_8k().replace(
+        scene="dam_break_model", rigid_v=(-1.0, 0.0, 0.0),
+        extra={"model": {"kind": "lattice", "cells": (3, 10, 10), "origin": (0.44, 0.0, 0.1)}})
+
+
+def paddle_1m() -> SolverConfig:
+    """dambreak_1m with a paddle of 3 x 60 x 60 boundary particles at
+    (1.02, 0, 0.2), beyond the fluid block's +x face, moving at 1 m/s along
+    -x into the collapsing column; 500 steps."""
+    return dambreak_1m().replace(
+        scene="dam_break_model", rigid_v=(-1.0, 0.0, 0.0),
+        extra={"model": {"kind": "lattice", "cells": (3, 60, 60), "origin": (1.02, 0.0, 0.2)}})
+
+
 PRESETS = {
     "reference": reference,
     "dambreak_8k": dambreak_8k,
@@ -169,6 +191,8 @@ PRESETS = {
     "torus_4m": torus_4m,
     "batch_256": batch_256,
     "learned_1m": learned_1m,
+    "paddle_8k": paddle_8k,
+    "paddle_1m": paddle_1m,
 }
 
 

@@ -59,15 +59,51 @@ __global__ void k_fill_f32(float *__restrict__ p, int n, float v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+// Boundary arrays into cell order; orig (null: the input is in original
+// order) gives each input slot's original index.
 __global__ void k_gather_boundary(const int *__restrict__ perm, int n, const float4 *__restrict__ a,
-                                  const float4 *__restrict__ b, float4 *__restrict__ ao,
-                                  float4 *__restrict__ bo, int *__restrict__ orig_o) {
+                                  const float4 *__restrict__ b, const int *__restrict__ orig,
+                                  float4 *__restrict__ ao, float4 *__restrict__ bo,
+                                  int *__restrict__ orig_o) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n) return;
   const int i = perm[f];
   ao[f] = a[i];
   bo[f] = b[i];
-  orig_o[f] = i;
+  orig_o[f] = orig ? orig[i] : i;
+}
+// set_rigid_v (IISPH.py:652-660) as launched at IISPH.py:1061-1081: in the
+// step that starts at sim_time 0, the moving part takes the rigid velocity.
+__global__ void k_set_rigid_v(float4 *__restrict__ bvel, const int *__restrict__ borig, int nb,
+                              int first, float vx, float vy, float vz,
+                              const DevState *__restrict__ st) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb || st->sim_time != 0.0 || borig[k] < first) return;
+  bvel[k] = make_float4(vx, vy, vz, bvel[k].w);
+}
+// update_rigid_x (IISPH.py:663-671): x += v * dt with the step's dt, after
+// the fluid drift. w (psi) is kept: the reference never recomputes psi.
+__global__ void k_rigid_drift(float4 *__restrict__ bpos, const float4 *__restrict__ bvel,
+                              const int *__restrict__ borig, int nb, int first,
+                              const DevState *__restrict__ st) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb || borig[k] < first) return;
+  const float dt = (float)st->dt;
+  float4 p = bpos[k];
+  const float4 v = bvel[k];
+  p.x = p.x + v.x * dt;
+  p.y = p.y + v.y * dt;
+  p.z = p.z + v.z * dt;
+  bpos[k] = p;
+}
+__global__ void k_copy_boundary(int n, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                const int *__restrict__ orig, float4 *__restrict__ pos_o,
+                                float4 *__restrict__ vel_o, int *__restrict__ orig_o) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  pos_o[k] = pos[k];
+  vel_o[k] = vel[k];
+  orig_o[k] = orig[k];
 }
 __global__ void k_gather_by_orig(const float *__restrict__ src_orig, const int *__restrict__ orig,
                                  int n, float *__restrict__ out) {
@@ -152,6 +188,13 @@ struct wsph_solver {
   // boundary
   float4 *bpos = nullptr, *bvel = nullptr;
   int *borig = nullptr;
+  // rigid motion of the boundary particles with original index >= rigid_first
+  // (-1: static boundary), and the buffers of the per-step boundary re-sort
+  int rigid_first = -1;
+  float rigid_v[3] = {0.0f, 0.0f, 0.0f};
+  float4 *rb_pos = nullptr, *rb_vel = nullptr;
+  int *rb_orig = nullptr, *rb_slot = nullptr, *rb_rank = nullptr, *rb_tmp = nullptr,
+      *rb_perm = nullptr;
   int final_dp = 0;
   bool searched = false, advected = false, solved = false;
   // timing: a ring of per-step event sets, read back lazily so that timed
@@ -420,6 +463,18 @@ static void drop_graphs(wsph_solver *s) {
   }
 }
 
+static void free_rigid(wsph_solver *s) {
+  drop_graphs(s);
+  s->rigid_first = -1;
+  dfree(s->rb_pos);
+  dfree(s->rb_vel);
+  dfree(s->rb_orig);
+  dfree(s->rb_slot);
+  dfree(s->rb_rank);
+  dfree(s->rb_tmp);
+  dfree(s->rb_perm);
+}
+
 static void free_fluid(wsph_solver *s) {
   drop_graphs(s);
   for (int k = 0; k < 2; ++k) {
@@ -476,6 +531,7 @@ int wsph_destroy(wsph_solver *s) {
   if (!s) return WSPH_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   free_fluid(s);
+  free_rigid(s);
   dfree(s->bpos);
   dfree(s->bvel);
   dfree(s->borig);
@@ -561,6 +617,7 @@ int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) 
   if (!s || nb < 0 || (nb > 0 && !xb)) return fail(WSPH_E_ARG, "bad boundary arguments");
   if (!s->wt) return fail(WSPH_E_ARG, "call wsph_set_tables first");
   drop_graphs(s);
+  free_rigid(s);
   dfree(s->bpos);
   dfree(s->bvel);
   dfree(s->borig);
@@ -600,8 +657,8 @@ int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) 
   }
   rc = sort_points(s, s->bg, p4, id, nb, sl, rk, tp, pm, bs, 1);
   if (!rc && nb > 0) {
-    k_gather_boundary<<<nblk(nb, 256), 256, 0, s->stream>>>(pm, nb, p4, v4, s->bpos, s->bvel,
-                                                            s->borig);
+    k_gather_boundary<<<nblk(nb, 256), 256, 0, s->stream>>>(pm, nb, p4, v4, nullptr, s->bpos,
+                                                            s->bvel, s->borig);
     if (bs) gather_i32(pm, nb, bs, s->bsid, s->stream);
     Bufs b = make_bufs(s);
     launch_boundary_psi(b, s->C, s->stream);
@@ -740,14 +797,69 @@ static void kernel_events(wsph_solver *s, hipEvent_t ev3[3]) {
   s->kev_rec[slot] = true;
 }
 
+// DYNAMIC_SCENE's boundary grid rebuild (IISPH.py:1415-1416): the moved
+// boundary, in its current cell order, is sorted again into cells, each
+// cell ordered by original index as at set-up.
+static int resort_boundary(wsph_solver *s) {
+  const int nb = s->nb;
+  if (s->rigid_first < 0 || nb == 0) return WSPH_OK;
+  k_copy_boundary<<<nblk(nb, 256), 256, 0, s->stream>>>(nb, s->bpos, s->bvel, s->borig, s->rb_pos,
+                                                         s->rb_vel, s->rb_orig);
+  int rc = sort_points(s, s->bg, s->rb_pos, s->rb_orig, nb, s->rb_slot, s->rb_rank, s->rb_tmp,
+                       s->rb_perm, nullptr, 1);
+  if (rc) return rc;
+  k_gather_boundary<<<nblk(nb, 256), 256, 0, s->stream>>>(s->rb_perm, nb, s->rb_pos, s->rb_vel,
+                                                          s->rb_orig, s->bpos, s->bvel, s->borig);
+  HIP_TRY(hipGetLastError());
+  return WSPH_OK;
+}
+
+static void launch_rigid_v(wsph_solver *s) {
+  if (s->rigid_first < 0 || s->nb == 0) return;
+  k_set_rigid_v<<<nblk(s->nb, 256), 256, 0, s->stream>>>(s->bvel, s->borig, s->nb, s->rigid_first,
+                                                         s->rigid_v[0], s->rigid_v[1],
+                                                         s->rigid_v[2], s->st);
+}
+
+static void launch_rigid_drift(wsph_solver *s) {
+  if (s->rigid_first < 0 || s->nb == 0) return;
+  k_rigid_drift<<<nblk(s->nb, 256), 256, 0, s->stream>>>(s->bpos, s->bvel, s->borig, s->nb,
+                                                         s->rigid_first, s->st);
+}
+
+int wsph_set_rigid(wsph_solver *s, int first, const float *v) {
+  if (!s) return fail(WSPH_E_ARG, "null solver");
+  if (!s->bpos) return fail(WSPH_E_ARG, "call wsph_set_boundary first");
+  if (first >= 0 && !v) return fail(WSPH_E_ARG, "null rigid velocity");
+  if (first >= 0 && s->nscenes > 0)
+    return fail(WSPH_E_UNSUPPORTED, "rigid boundary motion is not available with batched scenes");
+  free_rigid(s);
+  if (first < 0) return WSPH_OK;
+  const int nb = s->nb;
+  if (nb > 0) {
+    HIP_TRY(dalloc(&s->rb_pos, nb));
+    HIP_TRY(dalloc(&s->rb_vel, nb));
+    HIP_TRY(dalloc(&s->rb_orig, nb));
+    HIP_TRY(dalloc(&s->rb_slot, nb));
+    HIP_TRY(dalloc(&s->rb_rank, nb));
+    HIP_TRY(dalloc(&s->rb_tmp, nb));
+    HIP_TRY(dalloc(&s->rb_perm, nb));
+  }
+  s->rigid_first = first;
+  for (int k = 0; k < 3; ++k) s->rigid_v[k] = v[k];
+  return WSPH_OK;
+}
+
 int wsph_neighbor_search(wsph_solver *s) {
   if (!s || !s->pos[0]) return fail(WSPH_E_ARG, "no fluid set");
   if (s->timing && s->t_recorded - s->t_processed >= wsph_solver::kRing) timing_fold_one(s);
   launch_reset_state(make_bufs(s), s->stream);
   s->step_count++;
   rec(s, 0);
+  int rc = resort_boundary(s);
+  if (rc) return rc;
   const int c = s->cur, o = c ^ 1;
-  int rc = sort_points(s, s->fg, s->pos[c], s->orig[c], s->n, s->slot, s->rank, s->tmp, s->perm,
+  rc = sort_points(s, s->fg, s->pos[c], s->orig[c], s->n, s->slot, s->rank, s->tmp, s->perm,
                        s->sid[c], 0);
   if (rc) return rc;
   gather_fluid(s->perm, s->n, s->pos[c], s->vel[c], s->pprev[c], s->orig[c], s->pos[o], s->vel[o],
@@ -776,6 +888,7 @@ int wsph_predict_advection(wsph_solver *s) {
   const bool plain = s->P.vis_mu == 0.0 && s->P.xsph == 0.0;
   launch_density(b, s->C, plain, s->P.block, s->stream);
   if (!plain) launch_vadv(b, s->C, s->P.block, s->stream);
+  launch_rigid_v(s);
   launch_assemble(b, s->C, s->P.warm_start, s->P.block, s->stream);
   if (learned || s->log_pairs) {
     Weights W;
@@ -924,6 +1037,8 @@ int wsph_integrate(wsph_solver *s, wsph_step_stats *stats) {
   end.dt_max = s->P.dt_max;
   end.totals = s->totals;
   end.grid = s->persistent_grid;
+  // update_rigid_x reads the step's dt, which the clock update below replaces
+  launch_rigid_drift(s);
   launch_integrate(b, s->C, s->final_dp, s->P.warm_start, s->P.block, end, s->stream);
   s->have_pairs = s->log_pairs && s->n > 0;
   rec(s, 4);
@@ -949,7 +1064,9 @@ static int enqueue_front(wsph_solver *s, int c) {
   const wsph_params &P = s->P;
   s->cur = c;
   launch_reset_state(make_bufs(s), s->stream);
-  int rc = sort_points(s, s->fg, s->pos[c], s->orig[c], s->n, s->slot, s->rank, s->tmp, s->perm,
+  int rc = resort_boundary(s);
+  if (rc) return rc;
+  rc = sort_points(s, s->fg, s->pos[c], s->orig[c], s->n, s->slot, s->rank, s->tmp, s->perm,
                        s->sid[c], 0);
   if (rc) return rc;
   gather_fluid(s->perm, s->n, s->pos[c], s->vel[c], s->pprev[c], s->orig[c], s->pos[o], s->vel[o],
@@ -965,6 +1082,7 @@ static int enqueue_front(wsph_solver *s, int c) {
   const bool plain = P.vis_mu == 0.0 && P.xsph == 0.0;
   launch_density(b, s->C, plain, P.block, s->stream);
   if (!plain) launch_vadv(b, s->C, P.block, s->stream);
+  launch_rigid_v(s);
   launch_assemble(b, s->C, P.warm_start, P.block, s->stream);
   if (learned || s->log_pairs) {
     Weights W;
@@ -1077,6 +1195,8 @@ int wsph_get_field(wsph_solver *s, int field, void *dst) {
     case WSPH_F_FF_COUNT: launch_scatter(s->cnt_ff, 1, 1, 0, orig, n, dst, s->stream); break;
     case WSPH_F_FS_COUNT: launch_scatter(s->cnt_fs, 1, 1, 0, orig, n, dst, s->stream); break;
     case WSPH_F_PSI: launch_scatter(s->bpos, 4, 1, 3, s->borig, s->nb, dst, s->stream); break;
+    case WSPH_F_BX: launch_scatter(s->bpos, 4, 3, 0, s->borig, s->nb, dst, s->stream); break;
+    case WSPH_F_BV: launch_scatter(s->bvel, 4, 3, 0, s->borig, s->nb, dst, s->stream); break;
     case WSPH_F_CELL_KEY:
       if (n > 0) {
         k_keys4<<<nblk(n, 256), 256, 0, s->stream>>>(s->pos[c], n, s->C.cell_inv, s->P.grid_dims[0],

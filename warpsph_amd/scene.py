@@ -105,6 +105,30 @@ def dam_break(cfg: SolverConfig) -> Scene:
     return Scene(x32, xb32, np.zeros_like(xb32), xb32.shape[0], lo, hi)
 
 
+def model_points(spec: dict, spacing: float) -> np.ndarray:
+    """Boundary particles of a model, float64 [m, 3] (the reference's
+    ``load_model``, model.py:50-60, and its callers at IISPH.py:951-985).
+    ``spec["kind"] == "lattice"``: a solid lattice block of ``cells`` at
+    ``origin`` (x-major order, like the mesh samples' meshgrid order)."""
+    kind = spec.get("kind", "lattice")
+    d = float(spec.get("spacing", spacing))
+    if kind == "lattice":
+        return lattice_block(spec["cells"], d, spec.get("origin", (0.0, 0.0, 0.0)))
+    raise ValueError(f"unknown model kind {kind!r}")
+
+
+def dam_break_model(cfg: SolverConfig) -> Scene:
+    """``dam_break`` plus a model appended after the wall particles
+    (IISPH.py:951-989: walls first, so boundary_hide_n counts the walls);
+    ``cfg.extra["model"]`` describes it (see ``model_points``) and
+    ``cfg.rigid_v`` moves it."""
+    base = dam_break(cfg)
+    xm = _c32(model_points(cfg.extra["model"], cfg.spacing))
+    xb32 = _c32(np.concatenate([base.xb, xm]))
+    lo, hi = _bounds(base.x, xb32)
+    return Scene(base.x, xb32, np.zeros_like(xb32), base.xb.shape[0], lo, hi)
+
+
 def poisson_surface(shape: str, params: Sequence[float], radius: float, seed: int) -> np.ndarray:
     """Poisson-disk samples (float32 [n, 3]) of a plane rectangle
     (x0, z0, x1, z1, y) or a torus (cx, cy, cz, R, r), axis along y, with
@@ -184,6 +208,7 @@ GENERATORS = {
     "reference_box": reference_box,
     "dam_break": dam_break,
     "torus": torus_scene,
+    "dam_break_model": dam_break_model,
 }
 
 

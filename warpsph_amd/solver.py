@@ -76,11 +76,14 @@ class IISPH:
         N.check("wsph_set_tables", self._lib.wsph_set_tables(
             h, self.W_table.ctypes.data, self.W_table.shape[0],
             self.grad_W_table.ctypes.data, self.grad_W_table.shape[0]))
-        self.boundary_x = torch.from_numpy(np.ascontiguousarray(sc.xb, np.float32)).to(self.device)
-        self.boundary_v = torch.from_numpy(np.ascontiguousarray(sc.vb, np.float32)).to(self.device)
+        bx0 = torch.from_numpy(np.ascontiguousarray(sc.xb, np.float32)).to(self.device)
+        bv0 = torch.from_numpy(np.ascontiguousarray(sc.vb, np.float32)).to(self.device)
         _sync_torch()
         N.check("wsph_set_boundary", self._lib.wsph_set_boundary(
-            h, self.boundary_x.data_ptr(), self.boundary_v.data_ptr(), self.boundary_n))
+            h, bx0.data_ptr(), bv0.data_ptr(), self.boundary_n))
+        # a moving model (SceneType.PLANE / HAND, main.py:20, IISPH.py:1061-1081)
+        if cfg.rigid_v is not None:
+            self.set_rigid(self.boundary_hide_n, cfg.rigid_v)
         x0 = torch.from_numpy(np.ascontiguousarray(sc.x, np.float32)).to(self.device)
         _sync_torch()
         N.check("wsph_set_fluid", self._lib.wsph_set_fluid(h, x0.data_ptr(), None, self.n))
@@ -190,6 +193,15 @@ class IISPH:
         N.check("wsph_get_pairs", self._lib.wsph_get_pairs(self._h, f.data_ptr(), t.data_ptr()))
         return f, t
 
+    def set_rigid(self, first: int, v) -> None:
+        """Boundary particles from index ``first`` on move rigidly: velocity
+        ``v`` from the step at sim_time 0, positions advanced every step, the
+        boundary grid rebuilt every step (set_rigid_v / update_rigid_x /
+        DYNAMIC_SCENE, IISPH.py:652-671, 1061-1081, 1355-1364, 1415-1416).
+        ``first < 0`` makes the boundary static again."""
+        vv = (C.c_float * 3)(*[float(a) for a in (v if v is not None else (0.0, 0.0, 0.0))])
+        N.check("wsph_set_rigid", self._lib.wsph_set_rigid(self._h, int(first), vv))
+
     def set_pressure(self, p) -> None:
         """Caller-supplied start pressure for the next step (original order)."""
         torch = _torch()
@@ -202,7 +214,7 @@ class IISPH:
     def field(self, fid: int):
         torch = _torch()
         width = N.FIELD_WIDTH[fid]
-        count = self.boundary_n if fid == N.F_PSI else self.n
+        count = self.boundary_n if fid in N.BOUNDARY_FIELDS else self.n
         dtype = torch.int32 if fid in N.FIELD_IS_INT else torch.float32
         shape = (count, width) if width > 1 else (count,)
         out = torch.empty(shape, dtype=dtype, device=self.device)
@@ -239,6 +251,8 @@ class IISPH:
     ff_neighbor_num = property(lambda self: self.field(N.F_FF_COUNT))
     fs_neighbor_num = property(lambda self: self.field(N.F_FS_COUNT))
     boundary_phi = property(lambda self: self.field(N.F_PSI))
+    boundary_x = property(lambda self: self.field(N.F_BX))
+    boundary_v = property(lambda self: self.field(N.F_BV))
 
     @property
     def average_rho_error(self) -> float:
@@ -254,7 +268,7 @@ class IISPH:
         if self.sim_time - self.last_frame_time >= self.frame_dt:
             self.renderer.begin_frame(self.sim_time)
             self.renderer.render_points("fluid", self.x.cpu().numpy(), radius=self.cfg.spacing / 1.6)
-            rigid = self.scene.xb[self.boundary_hide_n:]
+            rigid = self.boundary_x[self.boundary_hide_n:].cpu().numpy()
             self.renderer.render_points("boundary", rigid, radius=self.cfg.spacing / 1.4)
             self.renderer.end_frame()
             self.last_frame_time = self.sim_time
