@@ -26,6 +26,14 @@ launches of every timed step's first Jacobi iteration.
 The CPU baseline times the float32 CPU oracle (one thread) on a bounded
 sample: the GPU's particle state at the end of the timed run, advanced a
 few steps.
+
+``--preset mesh_sample`` measures the model sampler instead (model.py:14-47):
+the lattice points inside a closed mesh, a 36,864-triangle torus
+(R = 0.3, r = 0.1) on its 400 x 100 x 400 lattice at 0.002 m, classified by
+wsph_mesh_inside with the mesh and lattice resident on the device; a "step"
+is one whole classification. Its roofline covers the longest of the five
+kernels (HIP event durations from the library), priced by the bytes it must
+move: each input read once, each output written once.
 """
 from __future__ import annotations
 
@@ -46,6 +54,8 @@ HBM_PEAK_GBS = 8000.0
 WARM = {}
 BYTES_PER_AP_EVAL = 92  # per fluid particle, J1 48 + J2 44
 BYTES_PER_STEP_FIXED = 264  # grid 84 + density 16 + v_adv 40 + assembly 48 + integrate 76
+MESH_SAMPLE = dict(mesh="torus", center=(0.0, 0.0, 0.0), R=0.3, r=0.1, nu=192, nv=96)
+MESH_SPACING = 0.002
 
 
 def parse():
@@ -108,7 +118,7 @@ def _nproc() -> int:
         return os.cpu_count() or 1
 
 
-def cpu_baseline(cfg, x, v, budget_s: float, threads: int = 1):
+def cpu_baseline(cfg, x, v, budget_s: float, threads: int = 1, boundary=None):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     from warpsph_amd import scene as scn
@@ -117,6 +127,8 @@ def cpu_baseline(cfg, x, v, budget_s: float, threads: int = 1):
     sc.x = x
     O_s = O.OracleSolver(cfg, sc)
     O_s.v = v.copy()
+    if boundary is not None:  # moved model: its positions, velocities and the clock
+        O_s.xb, O_s.vb, O_s.sim_time = boundary[0].copy(), boundary[1].copy(), boundary[2]
     t0 = time.perf_counter()
     steps = 0
     while True:
@@ -270,6 +282,100 @@ def bench_batch(args) -> None:
     print(json.dumps(line), flush=True)
 
 
+def bench_mesh(args) -> None:
+    """Model sampling: lattice points inside a closed mesh per second."""
+    import torch
+    from warpsph_amd import _native as N
+    from warpsph_amd import mesh as M
+    lib = N.load()
+    (v, t), axes = M.lattice_axes(MESH_SAMPLE, 1.0, MESH_SPACING)
+    nx, ny, nz = (a.size for a in axes)
+    npts, ncol, nt = nx * ny * nz, nx * ny, t.shape[0]
+    dev = torch.device("cuda:0")
+    dv, dtr = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (v, t))
+    dx, dy, dz = (torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev) for a in axes)
+    out = torch.zeros(npts, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+
+    def call():
+        N.check("wsph_mesh_inside", lib.wsph_mesh_inside(
+            dv.data_ptr(), v.shape[0], dtr.data_ptr(), nt, dx.data_ptr(), nx, dy.data_ptr(), ny,
+            dz.data_ptr(), nz, out.data_ptr()))
+
+    steps = max(1, min(args.steps, 50))
+    for _ in range(args.warmup):
+        call()
+    ms = np.zeros(5)
+    tm = (C.c_double * 7)()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+        N.check("wsph_mesh_timings", lib.wsph_mesh_timings(tm))
+        ms += np.array(tm[:5])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms /= steps
+    inside = int(out.sum().item())
+    pairs, crossings = tm[5], tm[6]
+    # algorithmic bytes per launch of each stage: every array it needs read
+    # once, every output written once (MeshTri 128 B, index/count 4 B, height 8 B)
+    stage_bytes = {
+        "setup": nt * (12 + 3 * 24 + 128),
+        "bin": nt * 128 + pairs * 4,
+        "count_pass": pairs * (4 + 128) + ncol * 4,
+        "fill_pass": pairs * (4 + 128) + ncol * 8 + crossings * 8,
+        "classify": ncol * 8 + crossings * 8 * 2 + npts,
+    }
+    names = list(stage_bytes)
+    dom = names[int(np.argmax(ms))]
+    achieved = stage_bytes[dom] / (ms[names.index(dom)] * 1e-3) / 1e9
+    line = {
+        "metric": "lattice points classified inside a closed mesh per second (model sampling)",
+        "value": npts * steps / elapsed, "unit": "points/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "mesh_sample", "mesh": "torus R 0.3 r 0.1, 192 x 96 quads",
+                   "triangles": nt, "lattice": [nx, ny, nz], "spacing": MESH_SPACING,
+                   "points_inside": inside},
+        "kernel_ms": dict(zip(names, ms.tolist())),
+        "triangle_tile_pairs": pairs, "column_crossings": crossings,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": {"setup": "k_mesh_setup", "bin": "k_mesh_bin",
+                                "count_pass": "k_mesh_columns<false>",
+                                "fill_pass": "k_mesh_columns<true>",
+                                "classify": "k_mesh_sort + k_mesh_mark"}[dom],
+                     "algorithmic_bytes_per_launch": stage_bytes[dom],
+                     "avg_launch_ms": float(ms[names.index(dom)])},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+        for threads, budget, key in ((1, args.cpu_seconds, "cpu_baseline"),
+                                     (_nproc(), args.cpu_seconds / 2, "cpu_baseline_all_cores")):
+            O.set_threads(threads)
+            stride = 1
+            while True:  # every stride-th column in x and z, all heights
+                sub = [axes[0][::stride], axes[1], axes[2][::stride]]
+                t1 = time.perf_counter()
+                O.mesh_inside(v, t, *[a[:1] for a in sub[:2]] + [sub[2][:1]])
+                per_point = time.perf_counter() - t1
+                if per_point * sub[0].size * sub[1].size * sub[2].size / max(1, threads) <= budget:
+                    break
+                stride *= 2
+            t1 = time.perf_counter()
+            want = O.mesh_inside(v, t, *sub)
+            el = time.perf_counter() - t1
+            got = out.cpu().numpy().reshape(nx, ny, nz)[::stride, :, ::stride]
+            line[key] = {"value": want.size / el, "unit": "points/s", "cores": threads,
+                         "kind": "port", "host_cpu": _cpu_model(), "nproc": _nproc(),
+                         "sample": f"every {stride}th lattice column in x and z "
+                                   f"({want.size} points, {el:.1f} s), brute-force oracle",
+                         "matches_gpu": bool(np.array_equal(got, want))}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy_mib:
@@ -281,6 +387,9 @@ def main():
     from warpsph_amd import config
     WARM.update(zero=config.WARM_ZERO, previous=config.WARM_PREVIOUS,
                 learned=config.WARM_LEARNED)
+    if args.preset == "mesh_sample":
+        bench_mesh(args)
+        return
     if config.preset(args.preset).scene == "batch":
         bench_batch(args)
         return
@@ -380,6 +489,9 @@ def main():
             else args.net}
     x_end = s.x.cpu().numpy()
     v_end = s.v.cpu().numpy()
+    b_end = None
+    if cfg.rigid_v is not None:
+        b_end = (s.boundary_x.cpu().numpy(), s.boundary_v.cpu().numpy(), s.get_time()[0])
     s.close()
     plain = {}
     for name in [c for c in args.compare.split(",") if c and c != args.warm_start]:
@@ -395,11 +507,12 @@ def main():
             line["speedup_vs_previous_start"] = r["value"] / plain["previous"]["value"]
     if not args.no_cpu_baseline:
         ccfg = base.replace(warm_start=config.WARM_PREVIOUS)
-        line["cpu_baseline"] = cpu_baseline(ccfg, x_end, v_end, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(ccfg, x_end, v_end, args.cpu_seconds, boundary=b_end)
         line["cpu_baseline"]["host_cpu"] = _cpu_model()
         line["cpu_baseline"]["nproc"] = _nproc()
         # the same oracle on every core the process may use
-        par = cpu_baseline(ccfg, x_end, v_end, args.cpu_seconds / 2, threads=_nproc())
+        par = cpu_baseline(ccfg, x_end, v_end, args.cpu_seconds / 2, threads=_nproc(),
+                           boundary=b_end)
         par["host_cpu"] = _cpu_model()
         par["nproc"] = _nproc()
         line["cpu_baseline_all_cores"] = par

@@ -272,6 +272,26 @@ enum { WSPH_SURF_PLANE = 0, WSPH_SURF_TORUS = 1 };
 int wsph_poisson_surface(int shape, const double *params, double radius, uint64_t seed,
                          int max_points, float *out, int *count);
 
+/* Lattice points inside a closed triangle mesh: the containment test of the
+ * reference's model sampler (_sample_points, model.py:14-47, which keeps the
+ * np.arange lattice points that trimesh's Trimesh.contains accepts).
+ * inside[(i * ny + j) * nz + k] = 1 when (ax[i], ay[j], az[k]) is inside,
+ * else 0: ray parity along +z, columns through an edge or vertex of the
+ * projected mesh resolved by the rasterisation top-left rule, so watertight
+ * meshes are classified consistently; a point on the surface counts the
+ * crossings strictly above it. ax, ay and az must be strictly ascending
+ * (WSPH_E_ARG otherwise), as np.arange lattices are. Device arrays:
+ * verts double[nv][3], tris int[nt][3] (vertex indices), ax/ay/az
+ * double[nx]/[ny]/[nz], inside uint8[nx * ny * nz]. Blocks until done. */
+int wsph_mesh_inside(const double *verts, int nv, const int *tris, int nt, const double *ax, int nx,
+                     const double *ay, int ny, const double *az, int nz, unsigned char *inside);
+/* Kernel durations (ms, HIP events) of the calling thread's last
+ * wsph_mesh_inside, out[0..4]: triangle set-up, binning of the triangles into
+ * 16 x 16 column tiles, crossing count pass, crossing fill pass, sort and
+ * classification; out[5] the triangle-tile pairs and out[6] the column
+ * crossings of that call. No reference counterpart. */
+int wsph_mesh_timings(double *out);
+
 /* ---- batched scenes (no reference counterpart) -------------------------
  * Many independent scenes solved together: call after wsph_create and before
  * the particle calls (wsph_set_boundary / wsph_set_fluid), whose

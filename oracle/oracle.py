@@ -100,6 +100,23 @@ def neighbors(xq, xs, h: float, dims, exclude_self: bool, cap: int = 128):
     return cnt, idx, dist
 
 
+def mesh_inside(verts, tris, ax, ay, az) -> np.ndarray:
+    """Boolean [nx, ny, nz] containment of the lattice points in the closed
+    mesh (orc_mesh_inside: per-point ray parity along +z, top-left rule)."""
+    v = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    ax, ay, az = (np.ascontiguousarray(a, np.float64) for a in (ax, ay, az))
+    out = np.zeros(ax.size * ay.size * az.size, np.uint8)
+    L = lib()
+    L.orc_mesh_inside.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                  C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+    rc = L.orc_mesh_inside(_ptr(v), v.shape[0], _ptr(t), t.shape[0], _ptr(ax), ax.size, _ptr(ay),
+                           ay.size, _ptr(az), az.size, _ptr(out))
+    if rc:
+        raise ValueError("triangle vertex index out of range")
+    return out.astype(bool).reshape(ax.size, ay.size, az.size)
+
+
 def boundary_psi(params, w_table, xb) -> np.ndarray:
     xb = _f32(xb, (-1, 3))
     wt = _f32(w_table)

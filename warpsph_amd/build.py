@@ -18,7 +18,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "libwsph.so"
 OBJ_DIR = PKG / "_build"
-SOURCES = ["capi.hip", "sph.hip", "grid.hip", "mlp.hip", "learn.hip", "scene_gen.cpp"]
+SOURCES = ["capi.hip", "sph.hip", "grid.hip", "mlp.hip", "learn.hip", "mesh.hip", "scene_gen.cpp"]
 ARCH = os.environ.get("WSPH_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

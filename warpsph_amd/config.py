@@ -184,6 +184,19 @@ def paddle_1m() -> SolverConfig:
         extra={"model": {"kind": "lattice", "cells": (3, 60, 60), "origin": (1.02, 0.0, 0.2)}})
 
 
+def wheel_8k() -> SolverConfig:
+    """dambreak_8k with a mesh-sampled model: a torus (R = 0.1, r = 0.04)
+    stood upright (rotated 90 degrees about x) at (0.6, 0.16, 0.5), filled
+    with boundary particles at the fluid spacing, moving into the fluid at
+    0.5 m/s along -x; 200 steps."""
+    rot = ((1.0, 0.0, 0.0), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0))
+    return dambreak_8k().replace(
+        scene="dam_break_model", rigid_v=(-0.5, 0.0, 0.0),
+        extra={"model": {"kind": "mesh", "mesh": {"mesh": "torus", "center": (0.0, 0.0, 0.0),
+                                                  "R": 0.1, "r": 0.04},
+                         "pos": (0.6, 0.16, 0.5), "rot": rot}})
+
+
 PRESETS = {
     "reference": reference,
     "dambreak_8k": dambreak_8k,
@@ -193,6 +206,7 @@ PRESETS = {
     "learned_1m": learned_1m,
     "paddle_8k": paddle_8k,
     "paddle_1m": paddle_1m,
+    "wheel_8k": wheel_8k,
 }
 
 

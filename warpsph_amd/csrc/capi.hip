@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "kernels.h"
@@ -192,6 +193,9 @@ struct wsph_solver {
   // (-1: static boundary), and the buffers of the per-step boundary re-sort
   int rigid_first = -1;
   float rigid_v[3] = {0.0f, 0.0f, 0.0f};
+  // the velocity set has not been launched yet: only the first step after
+  // wsph_set_rigid launches it (sim_time grows from then on), eagerly
+  bool rigid_v_pending = false;
   float4 *rb_pos = nullptr, *rb_vel = nullptr;
   int *rb_orig = nullptr, *rb_slot = nullptr, *rb_rank = nullptr, *rb_tmp = nullptr,
       *rb_perm = nullptr;
@@ -466,6 +470,7 @@ static void drop_graphs(wsph_solver *s) {
 static void free_rigid(wsph_solver *s) {
   drop_graphs(s);
   s->rigid_first = -1;
+  s->rigid_v_pending = false;
   dfree(s->rb_pos);
   dfree(s->rb_vel);
   dfree(s->rb_orig);
@@ -815,7 +820,8 @@ static int resort_boundary(wsph_solver *s) {
 }
 
 static void launch_rigid_v(wsph_solver *s) {
-  if (s->rigid_first < 0 || s->nb == 0) return;
+  if (s->rigid_first < 0 || s->nb == 0 || !s->rigid_v_pending) return;
+  s->rigid_v_pending = false;
   k_set_rigid_v<<<nblk(s->nb, 256), 256, 0, s->stream>>>(s->bvel, s->borig, s->nb, s->rigid_first,
                                                          s->rigid_v[0], s->rigid_v[1],
                                                          s->rigid_v[2], s->st);
@@ -846,7 +852,153 @@ int wsph_set_rigid(wsph_solver *s, int first, const float *v) {
     HIP_TRY(dalloc(&s->rb_perm, nb));
   }
   s->rigid_first = first;
+  s->rigid_v_pending = true;
   for (int k = 0; k < 3; ++k) s->rigid_v[k] = v[k];
+  return WSPH_OK;
+}
+
+// Kernel durations of the calling thread's last wsph_mesh_inside (ms):
+// triangle set-up, tile binning, crossing count pass, crossing fill pass,
+// classification.
+constexpr int kMeshStages = 5;
+// then the triangle-tile pairs and the column crossings of that call
+static thread_local double g_mesh_ms[kMeshStages + 2] = {};
+
+// Device buffers and events of one wsph_mesh_inside call, released on every
+// exit.
+struct MeshScratch {
+  hipStream_t st = nullptr;
+  hipEvent_t ev[2 * kMeshStages] = {};
+  MeshTri *tri = nullptr;
+  int *bad = nullptr, *ids = nullptr;
+  unsigned *cnt = nullptr, *off = nullptr, *fill = nullptr, *tile_cnt = nullptr,
+           *tile_off = nullptr, *tile_pos = nullptr;
+  int4 *items = nullptr;
+  double *z = nullptr;
+  ~MeshScratch() {
+    if (st) (void)hipStreamSynchronize(st);
+    dfree(tri);
+    dfree(bad);
+    dfree(ids);
+    dfree(cnt);
+    dfree(off);
+    dfree(fill);
+    dfree(items);
+    dfree(tile_cnt);
+    dfree(tile_off);
+    dfree(tile_pos);
+    dfree(z);
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+int wsph_mesh_timings(double *out) {
+  if (!out) return fail(WSPH_E_ARG, "null output");
+  for (int k = 0; k < kMeshStages + 2; ++k) out[k] = g_mesh_ms[k];
+  return WSPH_OK;
+}
+
+// Exclusive prefix sum in place; false when the total exceeds 32 bits.
+static bool scan_u32(std::vector<unsigned> &v, unsigned long long *total) {
+  unsigned long long t = 0;
+  for (auto &x : v) {
+    const unsigned c = x;
+    x = (unsigned)t;
+    t += c;
+  }
+  *total = t;
+  return t <= 0xffffffffull;
+}
+
+int wsph_mesh_inside(const double *verts, int nv, const int *tris, int nt, const double *ax, int nx,
+                     const double *ay, int ny, const double *az, int nz, unsigned char *inside) {
+  if (nv < 0 || nt < 0 || nx < 0 || ny < 0 || nz < 0) return fail(WSPH_E_ARG, "negative size");
+  const long ncol = (long)nx * ny, npts = ncol * nz;
+  for (int k = 0; k < kMeshStages + 2; ++k) g_mesh_ms[k] = 0.0;
+  if (npts == 0) return WSPH_OK;
+  if (!inside || !ax || !ay || !az) return fail(WSPH_E_ARG, "null lattice or output pointer");
+  if (nt > 0 && (!verts || !tris || nv < 1)) return fail(WSPH_E_ARG, "null mesh arrays");
+  const long ntiles = mesh_tiles(nx, ny);
+  MeshScratch m;
+  HIP_TRY(hipStreamCreate(&m.st));
+  for (auto &e : m.ev) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(dalloc(&m.cnt, ncol));
+  HIP_TRY(dalloc(&m.off, ncol));
+  HIP_TRY(dalloc(&m.fill, ncol));
+  HIP_TRY(dalloc(&m.tri, nt));
+  HIP_TRY(dalloc(&m.bad, 1));
+  HIP_TRY(dalloc(&m.tile_cnt, ntiles));
+  HIP_TRY(dalloc(&m.tile_off, ntiles));
+  HIP_TRY(dalloc(&m.tile_pos, ntiles));
+  HIP_TRY(hipMemsetAsync(m.bad, 0, sizeof(int), m.st));
+  HIP_TRY(hipMemsetAsync(m.tile_cnt, 0, sizeof(unsigned) * ntiles, m.st));
+  HIP_TRY(hipMemsetAsync(m.tile_pos, 0, sizeof(unsigned) * ntiles, m.st));
+  HIP_TRY(hipMemsetAsync(m.cnt, 0, sizeof(unsigned) * ncol, m.st));
+  HIP_TRY(hipMemsetAsync(m.fill, 0, sizeof(unsigned) * ncol, m.st));
+  launch_mesh_check_axes(ax, nx, m.bad, m.st);
+  launch_mesh_check_axes(ay, ny, m.bad, m.st);
+  launch_mesh_check_axes(az, nz, m.bad, m.st);
+  HIP_TRY(hipEventRecord(m.ev[0], m.st));
+  launch_mesh_setup(verts, nv, tris, nt, ax, nx, ay, ny, m.tri, m.tile_cnt, m.bad, m.st);
+  HIP_TRY(hipEventRecord(m.ev[1], m.st));
+  HIP_TRY(hipGetLastError());
+  std::vector<unsigned> h((size_t)ntiles);
+  int bad = 0;
+  HIP_TRY(hipMemcpyAsync(h.data(), m.tile_cnt, sizeof(unsigned) * ntiles, hipMemcpyDeviceToHost,
+                         m.st));
+  HIP_TRY(hipMemcpyAsync(&bad, m.bad, sizeof(int), hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipStreamSynchronize(m.st));
+  if (bad & 1) return fail(WSPH_E_ARG, "triangle vertex index outside [0, %d)", nv);
+  if (bad & 2) return fail(WSPH_E_ARG, "lattice axes must be strictly ascending");
+  const std::vector<unsigned> per_tile = h;
+  unsigned long long total = 0;
+  if (!scan_u32(h, &total) || total > 0x7fffffffull)
+    return fail(WSPH_E_CAPACITY, "%llu triangle-tile pairs", total);
+  g_mesh_ms[kMeshStages] = (double)total;
+  // work items of the column passes: each tile's triangles in chunks
+  const int chunk = mesh_chunk();
+  std::vector<int4> items;
+  for (long k = 0; k < ntiles; ++k)
+    for (unsigned f = 0; f < per_tile[k]; f += chunk)
+      items.push_back(make_int4((int)k, (int)(h[k] + f), (int)std::min<unsigned>(chunk, per_tile[k] - f), 0));
+  const int nitems = (int)items.size();
+  HIP_TRY(dalloc(&m.items, items.size()));
+  if (nitems > 0)
+    HIP_TRY(hipMemcpyAsync(m.items, items.data(), sizeof(int4) * items.size(),
+                           hipMemcpyHostToDevice, m.st));
+  HIP_TRY(hipMemcpyAsync(m.tile_off, h.data(), sizeof(unsigned) * ntiles, hipMemcpyHostToDevice,
+                         m.st));
+  HIP_TRY(dalloc(&m.ids, (size_t)total));
+  HIP_TRY(hipEventRecord(m.ev[2], m.st));
+  launch_mesh_bin(m.tri, nt, ny, m.tile_off, m.tile_pos, m.ids, m.st);
+  HIP_TRY(hipEventRecord(m.ev[3], m.st));
+  HIP_TRY(hipEventRecord(m.ev[4], m.st));
+  launch_mesh_columns(m.tri, m.ids, m.items, nitems, ax, nx, ay, ny, m.cnt, nullptr, nullptr,
+                      nullptr, m.st);
+  HIP_TRY(hipEventRecord(m.ev[5], m.st));
+  HIP_TRY(hipGetLastError());
+  h.resize((size_t)ncol);
+  HIP_TRY(hipMemcpyAsync(h.data(), m.cnt, sizeof(unsigned) * ncol, hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipStreamSynchronize(m.st));
+  if (!scan_u32(h, &total)) return fail(WSPH_E_CAPACITY, "%llu column crossings", total);
+  g_mesh_ms[kMeshStages + 1] = (double)total;
+  HIP_TRY(hipMemcpyAsync(m.off, h.data(), sizeof(unsigned) * ncol, hipMemcpyHostToDevice, m.st));
+  HIP_TRY(dalloc(&m.z, (size_t)total));
+  HIP_TRY(hipEventRecord(m.ev[6], m.st));
+  launch_mesh_columns(m.tri, m.ids, m.items, nitems, ax, nx, ay, ny, m.cnt, m.off, m.fill, m.z,
+                      m.st);
+  HIP_TRY(hipEventRecord(m.ev[7], m.st));
+  HIP_TRY(hipEventRecord(m.ev[8], m.st));
+  launch_mesh_classify(m.off, m.cnt, m.z, ncol, az, nz, inside, m.st);
+  HIP_TRY(hipEventRecord(m.ev[9], m.st));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(m.st));
+  for (int k = 0; k < kMeshStages; ++k) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, m.ev[2 * k], m.ev[2 * k + 1]) == hipSuccess) g_mesh_ms[k] = ms;
+  }
   return WSPH_OK;
 }
 
@@ -1082,7 +1234,6 @@ static int enqueue_front(wsph_solver *s, int c) {
   const bool plain = P.vis_mu == 0.0 && P.xsph == 0.0;
   launch_density(b, s->C, plain, P.block, s->stream);
   if (!plain) launch_vadv(b, s->C, P.block, s->stream);
-  launch_rigid_v(s);
   launch_assemble(b, s->C, P.warm_start, P.block, s->stream);
   if (learned || s->log_pairs) {
     Weights W;
@@ -1163,7 +1314,8 @@ static int graph_step(wsph_solver *s, wsph_step_stats *stats) {
 int wsph_step(wsph_solver *s, wsph_step_stats *stats) {
   int rc;
   if (!s) return fail(WSPH_E_ARG, "null solver");
-  if (s->P.graph_iters > 0 && s->P.jacobi_mode == WSPH_JACOBI_DEVICE && s->n > 0)
+  if (s->P.graph_iters > 0 && s->P.jacobi_mode == WSPH_JACOBI_DEVICE && s->n > 0 &&
+      !s->rigid_v_pending)
     return graph_step(s, stats);
   if ((rc = wsph_neighbor_search(s))) return rc;
   if ((rc = wsph_predict_advection(s))) return rc;

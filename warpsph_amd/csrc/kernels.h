@@ -131,6 +131,37 @@ int persistent_max_blocks(const Bufs &b, const SphConst &c, int block, int *bloc
 void launch_scatter(const void *src, int stride_floats, int ncomp, int src_off, const int *orig,
                     int n, void *dst, hipStream_t s);
 
+// ---- lattice points inside a closed mesh (mesh.hip) ------------------------
+struct MeshTri {          // a triangle with its xy projection's data
+  double x[3], y[3], z[3];
+  double lo[2], hi[2];    // xy bounding box
+  int orient;             // +1 counter-clockwise, -1 clockwise, 0 degenerate
+  int tiles[4];           // column tiles covered: i-tiles [0]..[1], j-tiles [2]..[3]
+};
+// number of 16 x 16 column tiles of an nx x ny lattice
+long mesh_tiles(int nx, int ny);
+// *bad |= 2 unless a[0] < a[1] < ... < a[n - 1]
+void launch_mesh_check_axes(const double *a, int n, int *bad, hipStream_t st);
+// triangles from vertex/index arrays with their tile ranges, and per-tile
+// triangle counts (tile_cnt zeroed by the caller); *bad |= 1 for an index
+// outside [0, nv)
+void launch_mesh_setup(const double *verts, int nv, const int *tris, int nt, const double *ax,
+                       int nx, const double *ay, int ny, MeshTri *out, unsigned *tile_cnt,
+                       int *bad, hipStream_t st);
+// triangle ids per tile at tile_off (tile_pos zeroed by the caller)
+void launch_mesh_bin(const MeshTri *tris, int nt, int ny, const unsigned *tile_off,
+                     unsigned *tile_pos, int *ids, hipStream_t st);
+// triangles per work item of the column passes
+int mesh_chunk();
+// per column (x_i, y_j), c = i * ny + j: crossing counts added to cnt (zbuf
+// null) or heights written at off[c] + fill cursor; one block per work item
+// {tile, first id, number of ids (<= mesh_chunk())}
+void launch_mesh_columns(const MeshTri *tris, const int *ids, const int4 *items, int nitems,
+                         const double *ax, int nx, const double *ay, int ny, unsigned *cnt,
+                         const unsigned *off, unsigned *fill, double *zbuf, hipStream_t st);
+void launch_mesh_classify(const unsigned *off, const unsigned *cnt, double *zbuf, long ncol,
+                          const double *az, int nz, unsigned char *inside, hipStream_t st);
+
 // Jacobi stop rule shared by the host loop and the persistent kernel
 // (IISPH.py:1200-1207): returns 1 to continue after iteration k.
 __host__ __device__ inline int jacobi_continue(int k, unsigned long long sum, unsigned cnt,

@@ -109,11 +109,19 @@ def model_points(spec: dict, spacing: float) -> np.ndarray:
     """Boundary particles of a model, float64 [m, 3] (the reference's
     ``load_model``, model.py:50-60, and its callers at IISPH.py:951-985).
     ``spec["kind"] == "lattice"``: a solid lattice block of ``cells`` at
-    ``origin`` (x-major order, like the mesh samples' meshgrid order)."""
+    ``origin`` (x-major order, like the mesh samples' meshgrid order);
+    ``"mesh"``: the lattice points inside ``spec["mesh"]`` (an OBJ path or a
+    procedural mesh, see ``warpsph_amd.mesh``) at ``scale``, rotated by
+    ``rot`` and moved to ``pos`` (sampled on the GPU)."""
     kind = spec.get("kind", "lattice")
     d = float(spec.get("spacing", spacing))
     if kind == "lattice":
         return lattice_block(spec["cells"], d, spec.get("origin", (0.0, 0.0, 0.0)))
+    if kind == "mesh":
+        from . import mesh
+        return mesh.load_model(spec["mesh"], spec.get("scale", 1.0),
+                               np.asarray(spec.get("pos", (0.0, 0.0, 0.0)), np.float64),
+                               np.asarray(spec.get("rot", np.eye(3)), np.float64), d)
     raise ValueError(f"unknown model kind {kind!r}")
 
 
