@@ -189,6 +189,7 @@ struct wsph_solver {
   // boundary
   float4 *bpos = nullptr, *bvel = nullptr;
   int *borig = nullptr;
+  unsigned *bnear = nullptr;  // boundary slots near a boundary particle (static boundary)
   // rigid motion of the boundary particles with original index >= rigid_first
   // (-1: static boundary), and the buffers of the per-step boundary re-sort
   int rigid_first = -1;
@@ -321,6 +322,8 @@ static Bufs make_bufs(wsph_solver *s) {
   b.bvel = s->bvel;
   b.bstart = s->bg.start;
   b.bcount = s->bg.count;
+  // a moving boundary is re-sorted every step; its queries search every cell
+  b.bnear = s->rigid_first < 0 ? s->bnear : nullptr;
   b.wt = s->wt;
   b.gt = s->gt;
   b.lds_tables = s->lds_tables;
@@ -469,6 +472,8 @@ static void drop_graphs(wsph_solver *s) {
 
 static void free_rigid(wsph_solver *s) {
   drop_graphs(s);
+  // the boundary may have moved: its near-boundary marks no longer hold
+  if (s->rigid_first >= 0) dfree(s->bnear);
   s->rigid_first = -1;
   s->rigid_v_pending = false;
   dfree(s->rb_pos);
@@ -540,6 +545,7 @@ int wsph_destroy(wsph_solver *s) {
   dfree(s->bpos);
   dfree(s->bvel);
   dfree(s->borig);
+  dfree(s->bnear);
   dfree(s->bsid);
   dfree(s->d_scenes);
   dfree(s->sc_sum);
@@ -626,10 +632,12 @@ int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) 
   dfree(s->bpos);
   dfree(s->bvel);
   dfree(s->borig);
+  dfree(s->bnear);
   s->nb = nb;
   HIP_TRY(dalloc(&s->bpos, nb));
   HIP_TRY(dalloc(&s->bvel, nb));
   HIP_TRY(dalloc(&s->borig, nb));
+  HIP_TRY(dalloc(&s->bnear, s->bg.ncell));
   float4 *p4 = nullptr, *v4 = nullptr;
   int *id = nullptr, *sl = nullptr, *rk = nullptr, *tp = nullptr, *pm = nullptr, *bs = nullptr;
   int rc = WSPH_OK;
@@ -667,6 +675,8 @@ int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) 
     if (bs) gather_i32(pm, nb, bs, s->bsid, s->stream);
     Bufs b = make_bufs(s);
     launch_boundary_psi(b, s->C, s->stream);
+    fill_u32(s->bnear, 0u, s->bg.ncell, s->stream);
+    launch_mark_bnear(b, s->C, s->bnear, s->stream);
     if (hipGetLastError() != hipSuccess) rc = fail(WSPH_E_HIP, "boundary psi launch failed");
   }
   hipError_t e = hipStreamSynchronize(s->stream);

@@ -82,6 +82,9 @@ struct Bufs {
   float4 *bpos;     // x, y, z, psi
   float4 *bvel;
   const unsigned *bstart, *bcount;
+  // per boundary slot: nonzero when a boundary particle's cell lies within 2
+  // cells on every axis (null: not known, every query searches)
+  const unsigned *bnear;
   // tables
   const float *wt, *gt;
   bool lds_tables;
@@ -90,6 +93,8 @@ struct Bufs {
 
 void launch_boundary_psi(const Bufs &b, const SphConst &c, hipStream_t s);
 void launch_lists(const Bufs &b, const SphConst &c, int block, hipStream_t s);
+// mark the boundary slots near boundary particles (bnear zeroed by the caller)
+void launch_mark_bnear(const Bufs &b, const SphConst &c, unsigned *bnear, hipStream_t s);
 void launch_density(const Bufs &b, const SphConst &c, bool fuse_vadv, int block, hipStream_t s);
 void launch_vadv(const Bufs &b, const SphConst &c, int block, hipStream_t s);
 // start: WSPH_WARM_* ; writes dp[0] = (d_ii, p1) for a zero start, else (d_ii, p0)

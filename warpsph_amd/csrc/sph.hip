@@ -320,6 +320,23 @@ struct ListWriter {
   }
 };
 
+// Whether particle p can have boundary candidates: its query range lies within
+// 2 cells of its own cell on every axis, and the slots within 2 cells of every
+// boundary particle's cell are marked (hashing is modular in the cell
+// coordinate, so a candidate always marks the query's own slot). Unknown
+// marks, or coordinates where the hash clamps, count as near.
+__device__ __forceinline__ bool near_boundary(const Bufs &b, const SphConst &c, const GridRef &g,
+                                              const float4 &p) {
+  if (!b.bnear) return true;
+  const int cx = (int)(p.x * c.cell_inv), cy = (int)(p.y * c.cell_inv), cz = (int)(p.z * c.cell_inv);
+  const int lim = -(1 << 20) + 8;
+  if (cx < lim || cy < lim || cz < lim) return true;
+  const int hx = hash_axis(cx, g.dims[0]), hy = hash_axis(cy, g.dims[1]), hz = hash_axis(cz, g.dims[2]);
+  const int d0 = g.dims[0], d1 = g.dims[1];
+  const int sl = c.layout == WSPH_LAYOUT_MORTON ? (int)morton3(hx, hy, hz) : hz * (d0 * d1) + hy * d0 + hx;
+  return b.bnear[(g.start - b.bstart) + sl] != 0u;
+}
+
 __global__ void k_lists(Bufs b, SphConst c) {
   int *ring = reinterpret_cast<int *>(g_smem) + (threadIdx.x >> 6) * (kListRing * 64);
   const int i = particle_index();
@@ -338,8 +355,8 @@ __global__ void k_lists(Bufs b, SphConst c) {
       [&]() { wf.row_done(); });
   wf.finish();
   ListWriter wb{ring, b.nbr_fs, n, i, (int)(threadIdx.x & 63), b.cap_fs, live};
-  if (b.nb > 0) {
-    const GridRef bg = boundary_grid(b, c, scene);
+  const GridRef bg = boundary_grid(b, c, scene);
+  if (b.nb > 0 && __ballot(live && near_boundary(b, c, bg, p))) {
     for_query_rows(
         p.x, p.y, p.z, live, c, bg.dims, bg.start, bg.count,
         [&](int s0, int e0) {
@@ -360,6 +377,30 @@ __global__ void k_lists(Bufs b, SphConst c) {
     atomicMax(&b.st->max_fs, ms);
     if (mf > b.cap_ff) atomicOr(&b.st->status, (int)ST_FF_OVERFLOW);
     if (ms > b.cap_fs) atomicOr(&b.st->status, (int)ST_FS_OVERFLOW);
+  }
+}
+
+// Marks of near_boundary: every slot within 2 cells (each axis) of a boundary
+// particle's cell.
+__global__ void k_mark_bnear(Bufs b, SphConst c, unsigned *__restrict__ bnear) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b.nb) return;
+  const float4 p = b.bpos[k];
+  const GridRef g = boundary_grid(b, c, b.scenes ? b.bsid[k] : 0);
+  const long base = g.start - b.bstart;
+  const int d0 = g.dims[0], d1 = g.dims[1], d2 = g.dims[2];
+  const int cx = (int)(p.x * c.cell_inv), cy = (int)(p.y * c.cell_inv), cz = (int)(p.z * c.cell_inv);
+  for (int dz = -2; dz <= 2; ++dz) {
+    const int hz = hash_axis(cz + dz, d2);
+    for (int dy = -2; dy <= 2; ++dy) {
+      const int hy = hash_axis(cy + dy, d1);
+      for (int dx = -2; dx <= 2; ++dx) {
+        const int hx = hash_axis(cx + dx, d0);
+        const int sl = c.layout == WSPH_LAYOUT_MORTON ? (int)morton3(hx, hy, hz)
+                                                      : hz * (d0 * d1) + hy * d0 + hx;
+        bnear[base + sl] = 1u;
+      }
+    }
   }
 }
 
@@ -1327,6 +1368,10 @@ static inline size_t tab_bytes(const Bufs &b, const SphConst &c) {
 void launch_boundary_psi(const Bufs &b, const SphConst &c, hipStream_t s) {
   if (b.nb == 0) return;
   DISPATCH_LDS(k_boundary_psi, nblocks(b.nb, 256), 256, tab_bytes(b, c), s, b, c);
+}
+
+void launch_mark_bnear(const Bufs &b, const SphConst &c, unsigned *bnear, hipStream_t s) {
+  if (b.nb > 0) k_mark_bnear<<<nblocks(b.nb, 256), 256, 0, s>>>(b, c, bnear);
 }
 
 void launch_lists(const Bufs &b, const SphConst &c, int block, hipStream_t s) {
