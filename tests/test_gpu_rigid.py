@@ -96,3 +96,32 @@ def test_rigid_clears_with_new_boundary():
     s.set_rigid(-1, None)  # static again
     s.step()
     assert_same("boundary x", s.boundary_x.cpu().numpy(), s.scene.xb)
+
+
+def test_reference_plane_scene_matches_oracle(orc, tmp_path, monkeypatch):
+    """The reference's PLANE scene through wp_sph.solver.IISPH with
+    wp_sph.main.scene_type switched, at a coarse spacing, its model a box mesh
+    standing in for the missing plane.obj: the model moves at (2, 0, 0) and
+    the run follows the oracle bit for bit."""
+    _announce("rigid: reference PLANE scene with a stand-in model")
+    import wp_sph.main as wm
+    from wp_sph.solver import IISPH as RefIISPH
+    from warpsph_amd import mesh as M
+    v, t = M.box_mesh((0.0, 0.0, 0.0), (400.0, 100.0, 600.0))
+    lines = [f"v {a} {b} {c}" for a, b, c in v] + [f"f {a + 1} {b + 1} {c + 1}" for a, b, c in t]
+    (tmp_path / "plane.obj").write_text("\n".join(lines) + "\n")
+    monkeypatch.setattr(wm, "scene_type", wm.SceneType.PLANE)
+    monkeypatch.setattr(wm, "MODEL_DIR", tmp_path)
+    cfg = config.reference("PLANE", models_dir=str(tmp_path)).replace(spacing=0.25)
+    sc = scene.build(cfg)
+    assert sc.boundary_hide_n == 0 and sc.nb > 20
+    s = RefIISPH(stage_path=None, config=cfg, scene=sc, collect_stats=True)
+    o = orc.OracleSolver(cfg, sc)
+    for _ in range(6):
+        s.step()
+        info = o.step()
+    _compare(s, o, info, "PLANE")
+    assert np.all(s.boundary_x.cpu().numpy()[:, 0] > sc.xb[:, 0])
+    # without a config the class builds the scene scene_type names
+    s2 = RefIISPH(stage_path=None)
+    assert s2.cfg.rigid_v == (2.0, 0.0, 0.0) and s2.boundary_hide_n == 0

@@ -71,3 +71,25 @@ def test_batch_rejects_rigid_motion():
     cfg = config.paddle_8k()
     with pytest.raises(ValueError, match="rigid"):
         BatchIISPH([cfg, cfg])
+
+
+def test_reference_scene_types():
+    """The reference's scene types (main.py:13-53, IISPH.py:751-766, 951-985):
+    EMPTY is the default scene; PLANE has no walls and no gravity and moves its
+    model along +x; HAND's model is turned 160 degrees about z and moves up;
+    HOUSE keeps a static model."""
+    import numpy as np
+    assert config.reference("EMPTY") == config.reference()
+    assert scene.box_shell((8.0, 8.0, 8.0), 0.05, 0).shape == (0, 3)
+    pl = config.reference("PLANE", models_dir="/models")
+    assert pl.boundary_layers == 0 and pl.gravity == 0.0 and pl.rigid_v == (2.0, 0.0, 0.0)
+    assert pl.extra["model"]["mesh"] == "/models/plane.obj"
+    hand = config.reference("HAND")
+    R = np.array(hand.extra["model"]["rot"])
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-15) and abs(np.linalg.det(R) - 1) < 1e-15
+    assert np.allclose(np.array([1.0, 0.0, 0.0]) @ R,
+                       [np.cos(np.radians(160)), -np.sin(np.radians(160)), 0.0])
+    assert hand.rigid_v == (0.0, 0.1, 0.0)
+    assert config.reference("HOUSE").rigid_v is None
+    with pytest.raises(ValueError):
+        config.reference("CASTLE")

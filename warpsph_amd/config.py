@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+from pathlib import Path
 from dataclasses import dataclass, field
 from typing import Optional, Tuple
 
@@ -110,12 +111,61 @@ class SolverConfig:
         return dataclasses.replace(self, **kw)
 
 
-def reference() -> SolverConfig:
-    """The reference's own scene: SceneType.EMPTY, 4x4x4 m box, d = 0.05,
-    3 wall layers, fluid block from (0.02,0.02,0.02) to (0.98,0.4,0.55) of the
-    box, CFL dt, 8192/65536-sample tables, p0 = 0 (main.py:19-61,
-    IISPH.py:738-766)."""
-    return SolverConfig()
+#: the reference's scene types (main.py:13-53, IISPH.py:751-766, 951-985): box,
+#: gravity, fluid block as fractions of the box, wall layers, model (OBJ file,
+#: scale, position, rotation) and the model's rigid velocity
+REFERENCE_SCENES = {
+    "EMPTY": dict(box=(4.0, 4.0, 4.0), gravity=9.8, fluid=((0.02, 0.02, 0.02), (0.98, 0.4, 0.55)),
+                  layers=3, model=None, rigid_v=None),
+    "HOUSE": dict(box=(8.0, 10.0, 12.0), gravity=9.8,
+                  fluid=((0.7, 0.01, 0.8), (0.99, 0.5, 0.99)), layers=3,
+                  model=("house.obj", 2e-2, (3.0, 0.0, 4.0), None), rigid_v=None),
+    "PLANE": dict(box=(8.0, 8.0, 8.0), gravity=0.0,
+                  fluid=((0.45, 0.05, 0.05), (0.55, 0.95, 0.95)), layers=0,
+                  model=("plane.obj", 3e-3, (0.0, 4.0, 4.0),
+                         ((1.0, 0.0, 0.0), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0))),
+                  rigid_v=(2.0, 0.0, 0.0)),
+    "HAND": dict(box=(8.0, 50.0, 8.0), gravity=9.8,
+                 fluid=((0.48, 0.05, 0.48), (0.52, 0.99, 0.52)), layers=3,
+                 model=("hand.obj", 0.3, (4.5, 2.0, 1.0), "hand"), rigid_v=(0.0, 0.1, 0.0)),
+}
+
+
+def _hand_rotation():
+    """The HAND model's rotation, 160 degrees about z, computed with the same
+    numpy calls as the reference (IISPH.py:973-981)."""
+    import numpy as np
+    angle = np.radians(160)
+    return tuple(tuple(float(v) for v in row) for row in
+                 ((np.cos(angle), -np.sin(angle), 0), (np.sin(angle), np.cos(angle), 0), (0, 0, 1)))
+
+
+def reference(scene_type: str = "EMPTY", models_dir: Optional[str] = None) -> SolverConfig:
+    """The reference's scene of the given SceneType name (default EMPTY: 4x4x4 m
+    box, d = 0.05, 3 wall layers, fluid block from (0.02,0.02,0.02) to
+    (0.98,0.4,0.55) of the box), CFL dt, 8192/65536-sample tables, p0 = 0
+    (main.py:19-61, IISPH.py:738-766). HOUSE, PLANE and HAND add their model,
+    sampled from ``<models_dir>/<name>.obj`` (default: the repository's
+    ``models/`` folder, as main.py's MODEL_DIR); PLANE and HAND move it."""
+    try:
+        spec = REFERENCE_SCENES[scene_type]
+    except KeyError:
+        raise ValueError(f"unknown scene type {scene_type!r}; known: {sorted(REFERENCE_SCENES)}") \
+            from None
+    box = spec["box"]
+    lo, hi = spec["fluid"]
+    cfg = SolverConfig(box=box, gravity=spec["gravity"], boundary_layers=spec["layers"],
+                       fluid_min=tuple(b * f for b, f in zip(box, lo)),
+                       fluid_max=tuple(b * f for b, f in zip(box, hi)),
+                       rigid_v=spec["rigid_v"])
+    if spec["model"] is not None:
+        name, scale, pos, rot = spec["model"]
+        mdir = Path(models_dir) if models_dir else Path(__file__).resolve().parents[1] / "models"
+        model = {"kind": "mesh", "mesh": str(mdir / name), "scale": scale, "pos": pos}
+        if rot is not None:
+            model["rot"] = _hand_rotation() if rot == "hand" else rot
+        cfg = cfg.replace(extra={"model": model})
+    return cfg
 
 
 def dambreak_8k() -> SolverConfig:

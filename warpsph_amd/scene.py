@@ -62,7 +62,8 @@ def _bounds(*arrs: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
 
 
 def reference_box(cfg: SolverConfig) -> Scene:
-    """The reference's EMPTY scene (IISPH.py:763-766, 857-1000)."""
+    """The reference's scenes (IISPH.py:751-766, 857-1000): the wall shell, the
+    fluid block, and the model of ``cfg.extra["model"]`` if any."""
     bw, bh, bl = cfg.box
     min_point = cfg.fluid_min or (bw * 0.02, bh * 0.02, bl * 0.02)
     max_point = cfg.fluid_max or (bw * 0.98, bh * 0.4, bl * 0.55)
@@ -70,13 +71,16 @@ def reference_box(cfg: SolverConfig) -> Scene:
         if not (0 < min_point[a] < max_point[a] < cfg.box[a]):
             raise ValueError("Invalid range for rectangle")  # IISPH.py:931-939
     xb = box_shell(cfg.box, cfg.spacing, cfg.boundary_layers)
+    hide = xb.shape[0]
     axes = [np.arange(min_point[a], max_point[a], cfg.spacing) for a in range(3)]
     xx, yy, zz = np.meshgrid(*axes, indexing="ij")
     x = np.stack([xx.ravel(), yy.ravel(), zz.ravel()]).T
+    if cfg.extra.get("model"):  # the scene's model after the walls (IISPH.py:951-989)
+        xb = np.concatenate([xb, model_points(cfg.extra["model"], cfg.spacing)])
     x32 = _c32(x)
     xb32 = _c32(xb)
     lo, hi = _bounds(x32, xb32)
-    return Scene(x32, xb32, np.zeros_like(xb32), xb32.shape[0], lo, hi)
+    return Scene(x32, xb32, np.zeros_like(xb32), hide, lo, hi)
 
 
 def lattice_block(cells: Sequence[int], spacing: float,

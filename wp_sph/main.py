@@ -1,13 +1,15 @@
 """Physical and scene constants under the reference's names (wp_sph/main.py:6-61).
 
-Values equal the reference's defaults; they come from
-``warpsph_amd.config.SolverConfig`` so the solver and this module agree.
+Values equal the reference's for the chosen ``scene_type`` (box and gravity
+per scene, main.py:35-53); they come from ``warpsph_amd.config`` so the solver
+and this module agree. ``wp_sph.solver.IISPH`` builds the scene that
+``scene_type`` names, with models read from ``MODEL_DIR``.
 """
 import math
 from enum import Enum, auto
 from pathlib import Path
 
-from warpsph_amd.config import SolverConfig
+from warpsph_amd.config import REFERENCE_SCENES, SolverConfig
 
 MODEL_DIR = Path(__file__).parents[1] / "models"
 
@@ -26,7 +28,8 @@ class SceneType(Enum):
 scene_type = SceneType.EMPTY
 DYNAMIC_SCENE = scene_type in [SceneType.PLANE, SceneType.HAND]
 
-_cfg = SolverConfig()
+_cfg = SolverConfig(box=REFERENCE_SCENES[scene_type.name]["box"],
+                    gravity=REFERENCE_SCENES[scene_type.name]["gravity"])
 GRAVITY = _cfg.gravity
 DIAMETER = _cfg.spacing
 OMEGA = _cfg.omega
