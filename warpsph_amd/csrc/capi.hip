@@ -215,8 +215,9 @@ struct wsph_solver {
   long t_recorded = 0, t_processed = 0;
   double t_sum[kEv] = {0};
   int persistent_grid = 0;
-  // device-driven Jacobi loop: pinned copies of DevState::done_tag per chunk
-  int *done_flags = nullptr;
+  // device-driven Jacobi loop: pinned host copy of DevState::done_tag, stored
+  // by the device's stop test
+  int *done_host = nullptr;
   hipEvent_t chunk_ev[2] = {};
   int step_count = 0;
   // captured step graphs, one per fluid buffer parity at step start
@@ -335,6 +336,7 @@ static Bufs make_bufs(wsph_solver *s) {
     b.target = s->target;
   }
   b.err_part = s->err_part;
+  b.done_host = s->done_host;
   b.judge = learned && s->net && s->err_part;
   if (s->nscenes > 0) {
     b.scenes = s->d_scenes;
@@ -445,8 +447,8 @@ int wsph_create(const wsph_params *params, wsph_solver **out) {
   CREATE_TRY(hipMemsetAsync(s->st, 0, sizeof(DevState), s->stream));
   for (auto &e : s->ev) CREATE_TRY(hipEventCreate(&e));
   for (auto &e : s->chunk_ev) CREATE_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  CREATE_TRY(hipHostMalloc((void **)&s->done_flags, sizeof(int) * (kMaxIters + 8),
-                           hipHostMallocDefault));
+  CREATE_TRY(hipHostMalloc((void **)&s->done_host, sizeof(int), hipHostMallocDefault));
+  *s->done_host = -1;
   k_set_dt<<<1, 1, 0, s->stream>>>(s->st, P.dt, 0.0);
   CREATE_TRY(hipGetLastError());
   int rcg = alloc_grid(s->fg, P.grid_dims, P.cell_layout);
@@ -569,7 +571,7 @@ int wsph_destroy(wsph_solver *s) {
     if (e) (void)hipEventDestroy(e);
   for (auto &e : s->chunk_ev)
     if (e) (void)hipEventDestroy(e);
-  if (s->done_flags) (void)hipHostFree(s->done_flags);
+  if (s->done_host) (void)hipHostFree(s->done_host);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return WSPH_OK;
@@ -1063,8 +1065,11 @@ int wsph_predict_advection(wsph_solver *s) {
 }
 
 // Device-driven Jacobi loop: iterations queued in chunks, one chunk ahead of
-// the host's look at the device's stop flag. `queued` iterations (with their
-// done flags in done_flags[0 .. m0)) were enqueued already.
+// the host's look at the device's stop flag. `queued` iterations in m0 chunks
+// were enqueued already. The stop test stores the step's tag in the pinned
+// done_host when the loop ends; once a chunk's event has completed, everything
+// that chunk stored is visible to the host (a later chunk can only store the
+// same tag).
 static int device_loop(wsph_solver *s, const Bufs &b, int src, int queued, int m0) {
   const wsph_params &P = s->P;
   const int C = P.chunk_iters > 0 ? P.chunk_iters : 2;
@@ -1076,13 +1081,11 @@ static int device_loop(wsph_solver *s, const Bufs &b, int src, int queued, int m
     if (queued == 0) kernel_events(s, ev3);
     launch_jacobi_chunk(b, s->C, P, src, cnt, P.block, s->stream, ev3);
     queued += cnt;
-    HIP_TRY(hipMemcpyAsync(&s->done_flags[m], &s->st->done_tag, sizeof(int),
-                           hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipEventRecord(s->chunk_ev[m & 1], s->stream));
     if (queued >= max_it) break;
     if (m >= 1) {
       HIP_TRY(hipEventSynchronize(s->chunk_ev[(m - 1) & 1]));
-      if (__atomic_load_n(&s->done_flags[m - 1], __ATOMIC_ACQUIRE) == tag) break;
+      if (__atomic_load_n(s->done_host, __ATOMIC_ACQUIRE) == tag) break;
     }
   }
   launch_batch_finalize(b, s->stream);
@@ -1260,8 +1263,6 @@ static int enqueue_front(wsph_solver *s, int c) {
   const int max_it = P.force_iters > 0 ? P.force_iters : P.max_iters;
   const int K = P.graph_iters < max_it ? P.graph_iters : max_it;
   launch_jacobi_chunk(b, s->C, P, src, K, P.block, s->stream);
-  HIP_TRY(hipMemcpyAsync(&s->done_flags[0], &s->st->done_tag, sizeof(int), hipMemcpyDeviceToHost,
-                         s->stream));
   HIP_TRY(hipGetLastError());
   return WSPH_OK;
 }

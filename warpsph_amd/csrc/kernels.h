@@ -85,6 +85,7 @@ struct Bufs {
   // per boundary slot: nonzero when a boundary particle's cell lies within 2
   // cells on every axis (null: not known, every query searches)
   const unsigned *bnear;
+  int *done_host;  // pinned host memory: the stop test stores the finished step's tag
   // tables
   const float *wt, *gt;
   bool lds_tables;

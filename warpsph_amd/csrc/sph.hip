@@ -771,6 +771,7 @@ __global__ void k_residual_reduce(Bufs b, int nparts, int slot, int check, int s
         st->converged = e <= eta;
         st->final_src = src0 ^ (k & 1);
         st->done_tag = st->step_tag;
+        if (b.done_host) *(volatile int *)b.done_host = st->step_tag;
       }
     }
   }
@@ -884,6 +885,7 @@ __global__ void k_batch_stop(Bufs b, int src0, int min_it, int max_it, int force
     st->converged = conv;
     st->final_src = src0 ^ (k & 1);
     st->done_tag = st->step_tag;
+        if (b.done_host) *(volatile int *)b.done_host = st->step_tag;
   }
 }
 
