@@ -155,6 +155,52 @@ def test_reference_scene_one_step(orc):
     assert dt == min(cfg.cfl * cfg.spacing / max(float(info["v_max"]), cfg.inv_small), cfg.dt_max)
 
 
+def test_reference_scene_trajectory(orc):
+    """The reference's own configuration over several steps: its default
+    scene, p0 = 0, the full Jacobi loop and the CFL clock (IISPH.py:1367-1371;
+    dt stays at TIME_STEP_MAX while v_max < 8 m/s). The device's clock, the
+    particle state and every step's iteration count follow the oracle's bit
+    for bit."""
+    cfg = config.reference()
+    _announce("reference scene, 6 steps")
+    s = gpu_solver(cfg)
+    sc = scene.build(cfg)
+    O = orc.OracleSolver(cfg, sc)
+    for k in range(6):
+        s.step()
+        info = O.step()
+        st = s.step_stats()
+        assert st["iterations"] == info["iterations"], k
+        assert st["residual"] == info["residual"], k
+        assert s.get_time() == (O.sim_time, O.dt), k
+    assert_same("x", s.x.cpu().numpy(), O.x)
+    assert_same("v", s.v.cpu().numpy(), O.v)
+    assert_same("p", s.p.cpu().numpy(), O.p)
+
+
+def test_cfl_dt_trajectory(orc):
+    """dt = min(cfl * d / max(v_max, INV_SMALL), TIME_STEP_MAX) each step
+    (IISPH.py:1368-1370) on the reference's scene at d = 0.1 with a CFL
+    number small enough that the limit binds from the first step and dt
+    changes every step: clock, state and iteration counts bit-identical to
+    the oracle's."""
+    cfg = config.reference().replace(spacing=0.1, cfl=0.002, warm_start=config.WARM_PREVIOUS)
+    _announce("CFL dt, 15 steps")
+    s = gpu_solver(cfg)
+    sc = scene.build(cfg)
+    O = orc.OracleSolver(cfg, sc)
+    dts = []
+    for k in range(15):
+        s.step()
+        info = O.step()
+        assert s.step_stats()["iterations"] == info["iterations"], k
+        assert s.get_time() == (O.sim_time, O.dt), k
+        dts.append(O.dt)
+    assert len(set(dts)) > 10 and min(dts) < cfg.dt_max
+    assert_same("x", s.x.cpu().numpy(), O.x)
+    assert_same("p", s.p.cpu().numpy(), O.p)
+
+
 def test_viscosity_term_bitexact(orc):
     """The Laplacian viscosity term as written (IISPH.py:249-278) with VIS_MU != 0."""
     cfg = config.dambreak_8k().replace(vis_mu=0.5, xsph=0.0, force_iters=2)
