@@ -384,7 +384,7 @@ def main():
         calibrate_copy(args.calibrate_copy_mib)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "warpsph_amd runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     from warpsph_amd import config
     WARM.update(zero=config.WARM_ZERO, previous=config.WARM_PREVIOUS,

@@ -53,10 +53,10 @@ enum { WSPH_LAYOUT_LINEAR = 0, WSPH_LAYOUT_MORTON = 1 };
  * see wsph_get_pairs) feeds the network, evaluated inline on the matrix
  * cores; the start value is max(0, p_carry + P_ref * net), the carried value
  * where |a_ii / a_ref| < 0.05 (weakly coupled particles), 0 inside the guard
- * band. The previous step's pressure judges the network: after each step
+ * band. The previous step's pressure arbitrates the network: after each step
  * both start values are compared with the converged pressure, and when the
  * network's error was the larger one the next step starts from the carried
- * pressure instead (the network is still evaluated and judged). */
+ * pressure instead (the network is still evaluated and compared). */
 enum { WSPH_WARM_ZERO = 0, WSPH_WARM_PREVIOUS = 1, WSPH_WARM_EXTERNAL = 2,
        WSPH_WARM_LEARNED = 3 };
 /* dt: CFL rule of IISPH.py:1368-1370, or fixed */

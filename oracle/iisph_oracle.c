@@ -1,7 +1,7 @@
 /*
  * iisph_oracle.c -- CPU float32 restatement of the reference IISPH step.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker for libwsph.so.
+ * TEST INFRASTRUCTURE ONLY. This library is the oracle of libwsph.so.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
  * it, and only to compute expected values or the CPU baseline time; the
  * product path never calls it.

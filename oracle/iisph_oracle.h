@@ -1,6 +1,6 @@
 /*
  * iisph_oracle.h -- CPU float32 restatement of the reference IISPH step.
- * TEST INFRASTRUCTURE ONLY (the checker of libwsph.so); see iisph_oracle.c.
+ * TEST INFRASTRUCTURE ONLY (the oracle of libwsph.so); see iisph_oracle.c.
  */
 #ifndef IISPH_ORACLE_H
 #define IISPH_ORACLE_H

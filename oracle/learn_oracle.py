@@ -1,6 +1,6 @@
 """CPU restatement of the learned start's feature pass and start value.
 
-TEST INFRASTRUCTURE ONLY: the checker of warpsph_amd/csrc/learn.hip (and the
+TEST INFRASTRUCTURE ONLY: the oracle of warpsph_amd/csrc/learn.hip (and the
 feature seeds written by k_assemble). Only tests/ import this module. The
 learned start has no counterpart in the reference, which starts every solve
 from p = 0 (wp_sph/solver/IISPH.py:166-174).
@@ -50,7 +50,7 @@ def p_ref(cfg) -> float:
     return float(np.float32(cfg.rho0 * max(abs(cfg.gravity), 1.0) * 0.1))
 
 
-def judge_error(p0, pf, a_ii, cfg, dt) -> float:
+def arbiter_error(p0, pf, a_ii, cfg, dt) -> float:
     """sqrt(mean((w (p0 - pf))^2)), w = min(|a_ii / a_ref|, 1)."""
     w = np.minimum(np.abs(np.asarray(a_ii, np.float64) / a_ref(cfg, dt)), 1.0)
     e = w * (np.asarray(p0, np.float64) - np.asarray(pf, np.float64))

@@ -1,5 +1,5 @@
 /* mesh_oracle.c -- CPU oracle of the mesh containment test (TEST
- * INFRASTRUCTURE ONLY: the checker of wsph_mesh_inside, never shipped).
+ * INFRASTRUCTURE ONLY: the oracle of wsph_mesh_inside, never shipped).
  *
  * The reference fills a model with the np.arange lattice points that
  * trimesh 4.6.10 (a dependency absent here) reports inside the mesh

@@ -1,7 +1,7 @@
 """CPU restatement of the per-particle pressure predictor (8 -> 64 -> 64 -> 1,
 SiLU) in float64 numpy.
 
-TEST INFRASTRUCTURE ONLY: the checker of the BF16 matrix-core kernels in
+TEST INFRASTRUCTURE ONLY: the oracle of the BF16 matrix-core kernels in
 warpsph_amd/csrc/mlp.hip. Only tests/ import this module.
 
 The network has no counterpart in the reference (it replaces the reference's

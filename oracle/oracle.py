@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liborc.so).
 
 TEST INFRASTRUCTURE ONLY: the float32 CPU restatement of the reference IISPH
-step used as the checker of libwsph.so. Only tests/, __graft_entry__.smoke()
+step that the tests compare libwsph.so with. Only tests/, __graft_entry__.smoke()
 and bench.py's cpu_baseline leg import this module.
 """
 from __future__ import annotations

@@ -2,7 +2,7 @@
 
 The reference's demo (scenes/IISPH_demo.py:5, :46) imports ``warp`` only to
 select a device with ``wp.ScopedDevice(args.device)`` before it builds the
-solver. This build has no Warp: ``ScopedDevice`` selects the ROCm device
+solver. Warp is not used here: ``ScopedDevice`` selects the ROCm device
 through torch and nothing else is provided. A script in this folder finds
 this file first on ``sys.path``.
 """
@@ -25,7 +25,7 @@ class ScopedDevice:
             self._prev = torch.cuda.current_device()
             torch.cuda.set_device(torch.device(str(self.device)))
         elif self.device not in (None, "cuda"):
-            raise RuntimeError(f"device {self.device!r}: this build runs on ROCm GPUs only")
+            raise RuntimeError(f"device {self.device!r}: warpsph_amd runs on ROCm GPUs only")
         return self
 
     def __exit__(self, *exc):

@@ -1,5 +1,5 @@
 """Learned pressure start (csrc/learn.hip, the seeds of k_assemble, the
-judge in k_integrate / k_vmax_reduce) against the CPU restatements
+arbiter in k_integrate / k_vmax_reduce) against the CPU restatements
 oracle/learn_oracle.py and the IISPH oracle.
 
 * features and regression targets of a logged step: within 1e-5 relative
@@ -9,8 +9,8 @@ oracle/learn_oracle.py and the IISPH oracle.
 * a constant network gives the start max(0, carry + P_ref * b3): with that
   start the oracle reproduces the GPU trajectory bit for bit, and the
   recorded a_ii-weighted start errors equal the CPU's within 1e-5 relative;
-* a network that predicts badly is switched off by the judge, and is then
-  evaluated and judged every 8th step only.
+* a network that predicts badly is switched off by the arbiter, and is then
+  evaluated and compared every 8th step only.
 """
 import sys
 
@@ -50,7 +50,7 @@ def _evaluated(r):
 
 
 def _next_source(r):
-    """The judge: the next step starts from the network only if it was
+    """The arbiter: the next step starts from the network only if it was
     evaluated and its start was no worse than the carried pressure."""
     ok = _evaluated(r) and r["start_err_learned"] <= r["start_err_carry"]
     return config.WARM_LEARNED if ok else config.WARM_PREVIOUS
@@ -135,8 +135,8 @@ def test_constant_network_start_matches_oracle(orc):
         pf = info["p"]
         if _evaluated(hist[k]):
             # the GPU weights with float32 a_ii * k1 / dt^2: compare within 1e-5
-            en = LO.judge_error(net_start, pf, info["a_ii"], cfg, cfg.dt0)
-            ec = LO.judge_error(carry, pf, info["a_ii"], cfg, cfg.dt0)
+            en = LO.arbiter_error(net_start, pf, info["a_ii"], cfg, cfg.dt0)
+            ec = LO.arbiter_error(carry, pf, info["a_ii"], cfg, cfg.dt0)
             assert abs(hist[k]["start_err_learned"] - en) <= 1e-5 * max(en, 1.0)
             assert abs(hist[k]["start_err_carry"] - ec) <= 1e-5 * max(ec, 1.0)
         if k + 1 < steps:
@@ -148,9 +148,9 @@ def test_constant_network_start_matches_oracle(orc):
     assert_same("v", s.v.cpu().numpy(), o.v)
 
 
-def test_judge_switches_off_a_bad_network():
+def test_arbiter_switches_off_a_bad_network():
     cfg = _cfg().replace(warm_start=config.WARM_LEARNED)
-    _announce("learned: judge, bad network")
+    _announce("learned: arbiter, bad network")
     net = _const_net(-30.0)  # start max(0, carry - 30 P_ref) = 0 everywhere
     s = _solver(cfg, net)
     for _ in range(20):

@@ -211,7 +211,7 @@ def batch_256() -> SolverConfig:
 
 def learned_1m() -> SolverConfig:
     """dambreak_1m with the learned pressure start value (predictor network,
-    judged against the previous step's pressure); 500 steps."""
+    compared with the previous step's pressure); 500 steps."""
     return dambreak_1m().replace(warm_start=WARM_LEARNED)
 
 

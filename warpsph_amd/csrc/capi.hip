@@ -337,7 +337,7 @@ static Bufs make_bufs(wsph_solver *s) {
   }
   b.err_part = s->err_part;
   b.done_host = s->done_host;
-  b.judge = learned && s->net && s->err_part;
+  b.arbiter = learned && s->net && s->err_part;
   if (s->nscenes > 0) {
     b.scenes = s->d_scenes;
     b.sid = s->sid[s->cur];

@@ -35,7 +35,7 @@ constexpr int kMaxLdsTable = 16384; // floats per table kept in LDS (64 KiB)
 // particle with |a_ii / a_ref| < kWeakCoupling keeps the carried pressure
 // (its pressure barely moves the density and its value is unreliable).
 constexpr float kFeatClamp = 64.0f;
-// While the network is switched off it is evaluated (and judged) every
+// While the network is switched off it is evaluated (and compared) every
 // kRetryEvery-th step only.
 constexpr int kRetryEvery = 8;
 constexpr float kWeakCoupling = 0.05f;

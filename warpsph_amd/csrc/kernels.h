@@ -61,7 +61,7 @@ struct Bufs {
   float *feat;          // feature rows [n][8] (pair logging)
   float *target;        // regression targets [n] (pair logging)
   double *err_part;     // per-block (net, carry) squared start errors of k_integrate
-  int judge;            // learned start: compare both start values after the solve
+  int arbiter;            // learned start: compare both start values after the solve
   // batched scenes (scenes == null: a single scene)
   const SceneGrid *scenes;
   const int *sid;       // fluid scene id, sorted order

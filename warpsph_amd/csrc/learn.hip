@@ -11,7 +11,7 @@
 //   2. optionally writes the 8-feature row (training pairs),
 //   3. runs the network on the wave's 64 rows in registers (infer64: two
 //      32-row MFMA tiles, BF16 operands, FP32 accumulation),
-// A network switched off by the judge is evaluated every kRetryEvery-th step
+// A network switched off by the arbiter is evaluated every kRetryEvery-th step
 // only (DevState::net_eval records whether it ran).
 //   4. writes the start value p0 = max(0, carry + P_ref * y) (the carried
 //      value where |a_ii / a_ref| < kWeakCoupling, zero inside the a_ii guard

@@ -1016,10 +1016,10 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
     b.pfinal[i] = di.w;
     const float4 au = b.aux[i];
     const float a = au.y;
-    if (b.target || b.judge) {
+    if (b.target || b.arbiter) {
       const float carry = b.pprev[i];  // this step's carried start value
       if (b.target) b.target[i] = clamp_feat((di.w - carry) * c.inv_pref);
-      if (b.judge && b.st->net_eval) {
+      if (b.arbiter && b.st->net_eval) {
         // start errors weighted by min(|a_ii / a_ref|, 1): a particle's share
         // of the density residual scales with a_ii
         float wgt = fabsf(a * c.feat_k1 / (dt * dt));
@@ -1043,7 +1043,7 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
   for (int o = 32; o > 0; o >>= 1) {
     ke += __shfl_xor(ke, o, 64);
     ysum += __shfl_xor(ysum, o, 64);
-    if (b.judge) {
+    if (b.arbiter) {
       en += __shfl_xor(en, o, 64);
       ec += __shfl_xor(ec, o, 64);
     }
@@ -1076,7 +1076,7 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
     b.ymax_part[blockIdx.x] = my;
     b.ke_part[blockIdx.x] = k2;
     b.y_part[blockIdx.x] = y2;
-    if (b.judge) {
+    if (b.arbiter) {
       b.err_part[2 * blockIdx.x] = n2;
       b.err_part[2 * blockIdx.x + 1] = c2;
     }
@@ -1099,7 +1099,7 @@ __global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode, StepEnd end) {
     my = max(my, b.ymax_part[k]);
     ke += b.ke_part[k];
     ys += b.y_part[k];
-    if (b.judge) {
+    if (b.arbiter) {
       en += b.err_part[2 * k];
       ec += b.err_part[2 * k + 1];
     }
@@ -1153,10 +1153,10 @@ __global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode, StepEnd end) {
     if (start_mode == WSPH_WARM_LEARNED) {
       // net_off still holds the choice this step's feature pass made
       if (st->net_off) r.start_source = WSPH_WARM_PREVIOUS;
-      if (b.judge && b.n > 0 && st->net_eval) {
+      if (b.arbiter && b.n > 0 && st->net_eval) {
         r.err_net = sqrt(en / b.n);
         r.err_carry = sqrt(ec / b.n);
-        // the carried pressure judges the network for the next step
+        // the carried pressure is compared with the network for the next step
         st->net_off = en > ec ? 1 : 0;
       }
     }

@@ -105,7 +105,7 @@ class IISPH:
         # build has none
         self.previewer = None
         if preview:
-            print("preview window unavailable: no OpenGL previewer in this build")
+            print("preview window unavailable: warpsph_amd has no OpenGL previewer")
 
     # ---- lifecycle ------------------------------------------------------
     def close(self):
@@ -281,5 +281,5 @@ class IISPH:
 
     @property
     def window_closed(self) -> bool:
-        """IISPH.window_closed (IISPH.py:1496-1501): no window in this build."""
+        """IISPH.window_closed (IISPH.py:1496-1501): there is no window."""
         return False
