@@ -26,9 +26,15 @@ __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
 // row of accumulator register i for this lane
 __device__ __forceinline__ int drow(int lane, int i) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware reciprocal (one ulp; the operands
+// of every layer are rounded to bf16 anyway): the library is built with
+// correctly rounded division, which would cost a full division sequence per
+// activation.
+__device__ __forceinline__ float silu(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 __device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
   return s * (1.0f + z * (1.0f - s));
 }
 
