@@ -109,3 +109,20 @@ def test_batch_with_an_empty_scene():
     for _ in range(4):
         s0.step()
     assert_same("scene 0 x", b.split(b.x.cpu().numpy())[0], s0.x.cpu().numpy())
+
+
+def test_demo_script_runs(tmp_path):
+    """scenes/IISPH_demo.py (the reference's demo command line) runs the
+    reference scene for 40 steps in a child process and records the fluid
+    and the boundary model as USDA time samples."""
+    import subprocess
+    from pathlib import Path
+    _announce("demo script, 40 steps")
+    root = Path(__file__).resolve().parents[1]
+    stage = tmp_path / "demo.usda"
+    r = subprocess.run([sys.executable, str(root / "scenes" / "IISPH_demo.py"), "--num_frames", "40",
+                        "--stage_path", str(stage), "--no-compress"],
+                       capture_output=True, text=True, timeout=240, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    text = stage.read_text()
+    assert "fluid" in text and "timeSamples" in text
