@@ -170,6 +170,17 @@ int wsph_abi_version(void);
  * d2 < wsph_radius_sq_threshold(h). */
 float wsph_radius_sq_threshold(float h);
 
+/* Self-test of the kernels' rounding helpers on the current device (no
+ * counterpart in the reference, which leaves float32 division and sqrt to
+ * Warp's codegen). The neighbour loops replace the compiler's correctly
+ * rounded 1.0f / b, sqrtf and a / b (for many a over one b) by shorter exact
+ * sequences; this runs them against the compiler's versions and writes the
+ * mismatch counts to out[3] (host): out[0] reciprocal over all 2^32 float32
+ * patterns, out[1] square root over every non-negative float32 up to +inf,
+ * out[2] division over 2^29 pairs covering every significand of the divisor.
+ * All three are 0 on a correct device; the parity tests check that. */
+int wsph_selftest_arith(unsigned long long *out);
+
 /* Create a solver (replaces IISPH.__init__'s allocations, IISPH.py:770-805). */
 int wsph_create(const wsph_params *params, wsph_solver **out);
 /* Release every device buffer the solver allocated. */

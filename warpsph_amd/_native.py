@@ -105,6 +105,7 @@ SIGNATURES = [
     ("wsph_last_error", C.c_char_p, []),
     ("wsph_abi_version", _I, []),
     ("wsph_radius_sq_threshold", C.c_float, [C.c_float]),
+    ("wsph_selftest_arith", _I, [C.POINTER(C.c_ulonglong)]),
     ("wsph_create", _I, [C.POINTER(WsphParams), C.POINTER(_P)]),
     ("wsph_destroy", _I, [_P]),
     ("wsph_set_tables", _I, [_P, _P, _I, _P, _I]),

@@ -388,6 +388,21 @@ const char *wsph_last_error(void) { return g_err; }
 float wsph_radius_sq_threshold(float h) { return radius_sq_threshold(h); }
 int wsph_abi_version(void) { return WSPH_ABI_VERSION; }
 
+int wsph_selftest_arith(unsigned long long *out) {
+  if (!out) return fail(WSPH_E_ARG, "null argument");
+  unsigned long long *d = nullptr;
+  HIP_TRY(hipMalloc(&d, 3 * sizeof(unsigned long long)));
+  hipError_t e = hipMemset(d, 0, 3 * sizeof(unsigned long long));
+  if (e == hipSuccess) {
+    launch_selftest_arith(d, nullptr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, d, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+  hipFree(d);
+  HIP_TRY(e);
+  return WSPH_OK;
+}
+
 int wsph_create(const wsph_params *params, wsph_solver **out) {
   if (!params || !out) return fail(WSPH_E_ARG, "null argument");
   const wsph_params &P = *params;
@@ -628,6 +643,7 @@ static int sort_points(wsph_solver *s, GridBufs &g, const float4 *pos, const int
 
 int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) {
   if (!s || nb < 0 || (nb > 0 && !xb)) return fail(WSPH_E_ARG, "bad boundary arguments");
+  if (nb > kMaxParticles) return fail(WSPH_E_CAPACITY, "%d boundary particles (at most %d)", nb, kMaxParticles);
   if (!s->wt) return fail(WSPH_E_ARG, "call wsph_set_tables first");
   drop_graphs(s);
   free_rigid(s);
@@ -696,6 +712,7 @@ int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) 
 
 int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
   if (!s || n < 0 || (n > 0 && !x)) return fail(WSPH_E_ARG, "bad fluid arguments");
+  if (n > kMaxParticles) return fail(WSPH_E_CAPACITY, "%d fluid particles (at most %d)", n, kMaxParticles);
   if (!s->wt) return fail(WSPH_E_ARG, "call wsph_set_tables first");
   if (!s->bpos) return fail(WSPH_E_ARG, "call wsph_set_boundary first");
   if (s->nscenes > 0) {

@@ -46,6 +46,9 @@ __host__ __device__ inline float clamp_feat(float v) {
 // Batched solves: each scene has its own fluid and boundary hash grids,
 // stored one after another in the cell tables (slot = base + local slot).
 constexpr int kMaxScenes = 4096;
+// particles per solver (fluid, and boundary, each): gathers address element
+// j of a 16-byte array with the 32-bit byte offset j * 16
+constexpr int kMaxParticles = (1 << 28) - 1;
 struct SceneGrid {
   int fdims[3], bdims[3];
   int fbase, bbase;
@@ -151,31 +154,78 @@ __device__ __forceinline__ Range3 query_range(float px, float py, float pz, floa
   return q;
 }
 
+// ---- correctly rounded square root and reciprocal -------------------------
+// sqrtf(x) for x >= 0 (including +inf): the steps of the compiler's correctly
+// rounded expansion (inputs below 2^-96 scaled by 2^32, hardware estimate,
+// one-ulp correction each way from the fma residuals, result scaled back by
+// 2^-16) without its zero/inf/NaN class test, which these inputs do not need:
+// for 0 and +inf both residuals are NaN or zero and the estimate is kept.
+// Bit-identical to sqrtf on every non-negative float32 (checked exhaustively
+// on gfx950).
+__device__ __forceinline__ float sqrt_rn(float x) {
+  const bool small = x < 0x1.0p-96f;
+  const float xs = small ? x * 0x1.0p+32f : x;
+  float s = __builtin_amdgcn_sqrtf(xs);
+  const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+  const float su = __uint_as_float(__float_as_uint(s) + 1u);
+  const float ed = __builtin_fmaf(-sd, s, xs);
+  const float eu = __builtin_fmaf(-su, s, xs);
+  s = ed <= 0.0f ? sd : s;
+  s = eu > 0.0f ? su : s;
+  return small ? s * 0x1.0p-16f : s;
+}
+
+// 1.0f / b rounded to nearest: the hardware reciprocal estimate and one
+// Newton step with fma, e = 1 - b*y0 (exact), y = y0 + e*y0. Equal to the
+// correctly rounded division for every float32 b in [2^-126, 2^126) (checked
+// exhaustively on gfx950); every other b (negative, zero, a reciprocal
+// outside the normal range, inf, NaN) takes the division. One unsigned compare
+// of the bit pattern selects [2^-126, 2^126) = [0x00800000, 0x7e800000).
+__device__ __forceinline__ float rcp_rn(float b) {
+  const float y0 = __builtin_amdgcn_rcpf(b);
+  const float e = __builtin_fmaf(-b, y0, 1.0f);
+  float y = __builtin_fmaf(e, y0, y0);
+  if (__builtin_expect(__float_as_uint(b) - 0x00800000u >= 0x7e000000u, 0)) y = 1.0f / b;
+  return y;
+}
+
 // ---- kernel tables: spline_W / grad_spline_W (sph_funcs.py:92-118) --------
+// The table read is done for every r (the index is clamped to the table) and
+// the cut-off applied by a select, so a neighbour loop has no branch.
 __device__ __forceinline__ float spline_w(const float *T, int n, float scale, float h, float r) {
-  if (r >= h) return 0.0f;
   const float idx = r * scale;
   int lo = (int)idx;
   lo = lo < n - 2 ? lo : n - 2;
   const float a = idx - (float)lo;
-  return (1.0f - a) * T[lo] + a * T[lo + 1];
+  const float w = (1.0f - a) * T[lo] + a * T[lo + 1];
+  return r >= h ? 0.0f : w;
 }
 
 // grad of W at x_j - x_i = r, i.e. the table value (-W'/|r|) times r.
 __device__ __forceinline__ float3 grad_w(const float *G, int n, float scale, float h, float3 r) {
-  const float len = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
-  if (len >= h || len < 1e-10f) return make_float3(0.0f, 0.0f, 0.0f);
+  const float len = sqrt_rn(r.x * r.x + r.y * r.y + r.z * r.z);
   const float idx = len * scale;
   int lo = (int)idx;
   lo = lo < n - 2 ? lo : n - 2;
   const float a = idx - (float)lo;
   const float s = (1.0f - a) * G[lo] + a * G[lo + 1];
-  return make_float3(s * r.x, s * r.y, s * r.z);
+  const bool zero = len >= h || len < 1e-10f;
+  return make_float3(zero ? 0.0f : s * r.x, zero ? 0.0f : s * r.y, zero ? 0.0f : s * r.z);
 }
 
 __device__ __forceinline__ float dist3(float ax, float ay, float az, float bx, float by, float bz) {
   const float dx = ax - bx, dy = ay - by, dz = az - bz;
-  return sqrtf(dx * dx + dy * dy + dz * dz);
+  return sqrt_rn(dx * dx + dy * dy + dz * dz);
+}
+
+// Element j of a device array through a 32-bit byte offset from its base
+// (particle arrays hold at most 2^28 elements, so j * 16 fits), which lets a
+// gather use the scalar-base plus 32-bit-offset address form instead of
+// 64-bit address arithmetic per lane.
+template <typename T>
+__device__ __forceinline__ T ld_at(const T *__restrict__ base, int j) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                      (unsigned)j * (unsigned)sizeof(T));
 }
 
 __device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }

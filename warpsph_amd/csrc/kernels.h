@@ -96,6 +96,7 @@ void launch_boundary_psi(const Bufs &b, const SphConst &c, hipStream_t s);
 void launch_lists(const Bufs &b, const SphConst &c, int block, hipStream_t s);
 // mark the boundary slots near boundary particles (bnear zeroed by the caller)
 void launch_mark_bnear(const Bufs &b, const SphConst &c, unsigned *bnear, hipStream_t s);
+void launch_selftest_arith(unsigned long long *bad, hipStream_t s);
 void launch_density(const Bufs &b, const SphConst &c, bool fuse_vadv, int block, hipStream_t s);
 void launch_vadv(const Bufs &b, const SphConst &c, int block, hipStream_t s);
 // start: WSPH_WARM_* ; writes dp[0] = (d_ii, p1) for a zero start, else (d_ii, p0)

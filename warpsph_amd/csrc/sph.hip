@@ -443,10 +443,10 @@ __global__ void k_density(Bufs b, SphConst c, int fuse_vadv) {
   const float4 *__restrict__ bpos = b.bpos;
   float t1 = 0.0f, t2 = 0.0f;
   for_neighbors<kBatch>(
-      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return pos[j]; },
+      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return ld_at(pos, j); },
       [&](const float4 &o) { t1 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z)); });
   for_neighbors<kBatch>(
-      b.nbr_fs, b.n, i, b.cnt_fs[i], [&](int j, long) { return bpos[j]; },
+      b.nbr_fs, b.n, i, b.cnt_fs[i], [&](int j, long) { return ld_at(bpos, j); },
       [&](const float4 &o) { t2 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z)) * o.w; });
   const float rho = t1 * c.mass + t2 + W_OF(T, 0.0f) * c.mass;
   reinterpret_cast<float *>(&b.pos[i])[3] = rho;
@@ -471,7 +471,7 @@ __global__ void k_vadv(Bufs b, SphConst c) {
   if (c.vis_mu != 0.0f) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
     for_neighbors<kBatch>(
-        b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{pos[j], vel[j]}; },
+        b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{ld_at(pos, j), ld_at(vel, j)}; },
         [&](const F4x2 &v) {
           const float4 o = v.a, vj = v.b;
           const float3 xij = sub3(o, p);
@@ -485,7 +485,7 @@ __global__ void k_vadv(Bufs b, SphConst c) {
           az = az - gw.z * dd / o.w / r2;
         });
     for_neighbors<kBatch>(
-        b.nbr_fs, b.n, i, ns, [&](int j, long) { return bpos[j]; },
+        b.nbr_fs, b.n, i, ns, [&](int j, long) { return ld_at(bpos, j); },
         [&](const float4 &va_) {
           const F4x2 v{va_, va_};
           const float4 o = v.a;
@@ -509,10 +509,11 @@ __global__ void k_vadv(Bufs b, SphConst c) {
     // XSPH: v_adv += xsph * sum_j (v_j - v_i) * ((m / rho_j) * W_ij)
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     for_neighbors<kBatch>(
-        b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{pos[j], vel[j]}; },
+        b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{ld_at(pos, j), ld_at(vel, j)}; },
         [&](const F4x2 &v) {
           const float4 o = v.a, vj = v.b;
-          const float coef = c.mass / o.w * W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z));
+          const float coef =
+              div_by(c.mass, o.w, rcp_rn(o.w)) * W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z));
           ax = ax + (vj.x - vi.x) * coef;
           ay = ay + (vj.y - vi.y) * coef;
           az = az + (vj.z - vi.z) * coef;
@@ -547,7 +548,7 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
   float tr1 = 0.0f, tr2 = 0.0f;
   float dx1 = 0.0f, dy1 = 0.0f, dz1 = 0.0f, dx2 = 0.0f, dy2 = 0.0f, dz2 = 0.0f;
   for_neighbors<kBatch>(
-      b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{pos[j], vadv[j]}; },
+      b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{ld_at(pos, j), ld_at(vadv, j)}; },
       [&](const F4x2 &v) {
         const float3 gw = GW_OF(T, sub3(v.a, p));
         tr1 += dot3(make_float3(va.x - v.b.x, va.y - v.b.y, va.z - v.b.z), gw);
@@ -556,7 +557,7 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
         dz1 = dz1 - gw.z;
       });
   for_neighbors<kBatch>(
-      b.nbr_fs, b.n, i, ns, [&](int j, long) { return F4x2{bpos[j], bvel[j]}; },
+      b.nbr_fs, b.n, i, ns, [&](int j, long) { return F4x2{ld_at(bpos, j), ld_at(bvel, j)}; },
       [&](const F4x2 &v) {
         const float4 o = v.a, vb = v.b;
         const float3 gw = GW_OF(T, sub3(o, p));
@@ -571,7 +572,7 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
   const float diz = (dz1 * c.mass + dz2) * dt2 / ri2;
   float ta1 = 0.0f, ta2 = 0.0f;
   for_neighbors<kBatch>(
-      b.nbr_ff, b.n, i, nf, [&](int j, long) { return pos[j]; },
+      b.nbr_ff, b.n, i, nf, [&](int j, long) { return ld_at(pos, j); },
       [&](const float4 &va_) {
         const F4x2 v{va_, va_};
         const float3 gw = GW_OF(T, sub3(v.a, p));
@@ -581,7 +582,7 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
         ta1 += dot3(dd, gw);
       });
   for_neighbors<kBatch>(
-      b.nbr_fs, b.n, i, ns, [&](int j, long) { return bpos[j]; },
+      b.nbr_fs, b.n, i, ns, [&](int j, long) { return ld_at(bpos, j); },
       [&](const float4 &va_) {
         const F4x2 v{va_, va_};
         const float3 gw = GW_OF(T, sub3(v.a, p));
@@ -620,7 +621,7 @@ __device__ __forceinline__ void j1_particle(const Bufs &b, const SphConst &c, co
   const float yi = 1.0f / ri2;
   float t1 = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
   for_neighbors<kBatchJ1>(
-      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return F4x2{pos[j], dp[j]}; },
+      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return F4x2{ld_at(pos, j), ld_at(dp, j)}; },
       [&](const F4x2 &v) {
         const float4 o = v.a, dj = v.b;
         const float3 gw = GW_OF(T, sub3(o, p));
@@ -630,7 +631,7 @@ __device__ __forceinline__ void j1_particle(const Bufs &b, const SphConst &c, co
                                         div_by(gw.z * t3, ri2, yi) * pi - dj.z * dj.w);
         t1 += dot3(term, gw);
         const float rj2 = o.w * o.w;
-        const float yj = 1.0f / rj2;
+        const float yj = rcp_rn(rj2);
         sx = sx + div_by(-gw.x * t3, rj2, yj) * dj.w;
         sy = sy + div_by(-gw.y * t3, rj2, yj) * dj.w;
         sz = sz + div_by(-gw.z * t3, rj2, yj) * dj.w;
@@ -651,13 +652,13 @@ __device__ __forceinline__ void j2_particle(const Bufs &b, const SphConst &c, co
   const float4 *__restrict__ bpos = b.bpos;
   float t1 = 0.0f, t2 = 0.0f;
   for_neighbors<kBatchJ2>(
-      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return F4x2{pos[j], sap[j]}; },
+      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return F4x2{ld_at(pos, j), ld_at(sap, j)}; },
       [&](const F4x2 &v) {
         const float3 gw = GW_OF(T, sub3(v.a, p));
         t1 += dot3(make_float3(si.x - v.b.x, si.y - v.b.y, si.z - v.b.z), gw);
       });
   for_neighbors<kBatchJ2>(
-      b.nbr_fs, b.n, i, b.cnt_fs[i], [&](int j, long) { return bpos[j]; },
+      b.nbr_fs, b.n, i, b.cnt_fs[i], [&](int j, long) { return ld_at(bpos, j); },
       [&](const float4 &va_) {
         const F4x2 v{va_, va_};
         const float3 gw = GW_OF(T, sub3(v.a, p));
@@ -1370,6 +1371,44 @@ static inline size_t tab_bytes(const Bufs &b, const SphConst &c) {
 void launch_boundary_psi(const Bufs &b, const SphConst &c, hipStream_t s) {
   if (b.nb == 0) return;
   DISPATCH_LDS(k_boundary_psi, nblocks(b.nb, 256), 256, tab_bytes(b, c), s, b, c);
+}
+
+// ---- self-test of the rounding helpers -------------------------------------
+// bad[0]: float32 bit patterns b (all 2^32) where rcp_rn(b) != 1.0f / b.
+// bad[1]: non-negative b up to +inf where sqrt_rn(b) != sqrtf(b).
+// bad[2]: pairs where div_by(a, b, rcp_rn(b)) != a / b, over every significand
+//   of b at 64 exponents in [-28, 35], each with a pseudo-random normal a of
+//   exponent in [-27, 27] (536,870,912 pairs; quotients are normal).
+__global__ void k_selftest_arith(unsigned long long *bad) {
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  unsigned nr = 0, ns = 0, nd = 0;
+  for (unsigned long long u = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+       u < (1ull << 32); u += stride) {
+    const unsigned bits = (unsigned)u;
+    const float b = __uint_as_float(bits);
+    nr += __float_as_uint(rcp_rn(b)) != __float_as_uint(1.0f / b);
+    if (bits <= 0x7f800000u) ns += __float_as_uint(sqrt_rn(b)) != __float_as_uint(sqrtf(b));
+    if (bits < (1u << 29)) {
+      const unsigned e = 99u + (bits >> 23);  // 64 exponents
+      const float bb = __uint_as_float((e << 23) | (bits & 0x7fffffu));
+      unsigned h = bits * 0x9e3779b9u;
+      h ^= h >> 16;
+      h *= 0x85ebca6bu;
+      h ^= h >> 13;
+      const float a = __uint_as_float(((100u + (h >> 23) % 55u) << 23) | (h & 0x7fffffu));
+      nd += __float_as_uint(div_by(a, bb, rcp_rn(bb))) != __float_as_uint(a / bb);
+    }
+  }
+  const unsigned long long r = wave_sum_u64(nr), q = wave_sum_u64(ns), d = wave_sum_u64(nd);
+  if ((threadIdx.x & 63) == 0) {
+    if (r) atomicAdd(&bad[0], r);
+    if (q) atomicAdd(&bad[1], q);
+    if (d) atomicAdd(&bad[2], d);
+  }
+}
+
+void launch_selftest_arith(unsigned long long *bad, hipStream_t s) {
+  k_selftest_arith<<<4096, 256, 0, s>>>(bad);
 }
 
 void launch_mark_bnear(const Bufs &b, const SphConst &c, unsigned *bnear, hipStream_t s) {
