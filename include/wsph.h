@@ -175,10 +175,11 @@ float wsph_radius_sq_threshold(float h);
  * Warp's codegen). The neighbour loops replace the compiler's correctly
  * rounded 1.0f / b, sqrtf and a / b (for many a over one b) by shorter exact
  * sequences; this runs them against the compiler's versions and writes the
- * mismatch counts to out[3] (host): out[0] reciprocal over all 2^32 float32
+ * mismatch counts to out[4] (host): out[0] reciprocal over all 2^32 float32
  * patterns, out[1] square root over every non-negative float32 up to +inf,
- * out[2] division over 2^29 pairs covering every significand of the divisor.
- * All three are 0 on a correct device; the parity tests check that. */
+ * out[2] division over 2^29 pairs covering every significand of the divisor,
+ * out[3] the neighbour-pair square root (exact from 2^-96 up, below 2^-47
+ * under it). All four are 0 on a correct device; the parity tests check that. */
 int wsph_selftest_arith(unsigned long long *out);
 
 /* Create a solver (replaces IISPH.__init__'s allocations, IISPH.py:770-805). */

@@ -21,6 +21,7 @@ def test_rounding_helpers_match_compiler():
     lib = N.load()
     import torch
     torch.cuda.init()
-    out = (ctypes.c_ulonglong * 3)()
+    out = (ctypes.c_ulonglong * 4)()
     N.check("wsph_selftest_arith", lib.wsph_selftest_arith(out))
-    assert list(out) == [0, 0, 0], f"mismatches: reciprocal {out[0]}, sqrt {out[1]}, division {out[2]}"
+    assert list(out) == [0, 0, 0, 0], (
+        f"mismatches: reciprocal {out[0]}, sqrt {out[1]}, division {out[2]}, pair sqrt {out[3]}")

@@ -167,6 +167,7 @@ struct wsph_solver {
   int n = 0, nb = 0, cap_n = 0;
   float *wt = nullptr, *gt = nullptr;
   bool lds_tables = false;
+  int lds_per_cu = 0;
   DevState *st = nullptr;
   DevState *st_host = nullptr;   // pinned mirror
   long long *totals = nullptr;   // device counters
@@ -328,6 +329,7 @@ static Bufs make_bufs(wsph_solver *s) {
   b.wt = s->wt;
   b.gt = s->gt;
   b.lds_tables = s->lds_tables;
+  b.lds_per_cu = s->lds_per_cu;
   b.st = s->st;
   const bool learned = s->P.warm_start == WSPH_WARM_LEARNED;
   if (learned || s->log_pairs) b.seed = s->seed;
@@ -391,13 +393,13 @@ int wsph_abi_version(void) { return WSPH_ABI_VERSION; }
 int wsph_selftest_arith(unsigned long long *out) {
   if (!out) return fail(WSPH_E_ARG, "null argument");
   unsigned long long *d = nullptr;
-  HIP_TRY(hipMalloc(&d, 3 * sizeof(unsigned long long)));
-  hipError_t e = hipMemset(d, 0, 3 * sizeof(unsigned long long));
+  HIP_TRY(hipMalloc(&d, 4 * sizeof(unsigned long long)));
+  hipError_t e = hipMemset(d, 0, 4 * sizeof(unsigned long long));
   if (e == hipSuccess) {
     launch_selftest_arith(d, nullptr);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpy(out, d, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(out, d, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
   hipFree(d);
   HIP_TRY(e);
   return WSPH_OK;
@@ -453,6 +455,9 @@ int wsph_create(const wsph_params *params, wsph_solver **out) {
       return r_;                                                                             \
     }                                                                                        \
   } while (0)
+  int dev = 0;
+  CREATE_TRY(hipGetDevice(&dev));
+  CREATE_TRY(hipDeviceGetAttribute(&s->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
   CREATE_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   CREATE_TRY(dalloc(&s->st, 1));
   CREATE_TRY(hipHostMalloc((void **)&s->st_host, sizeof(DevState), hipHostMallocDefault));

@@ -201,16 +201,36 @@ __device__ __forceinline__ float spline_w(const float *T, int n, float scale, fl
   return r >= h ? 0.0f : w;
 }
 
-// grad of W at x_j - x_i = r, i.e. the table value (-W'/|r|) times r.
-__device__ __forceinline__ float3 grad_w(const float *G, int n, float scale, float h, float3 r) {
-  const float len = sqrt_rn(r.x * r.x + r.y * r.y + r.z * r.z);
+// The square root of grad_w_pair: the steps of sqrt_rn without the range
+// scaling, so it equals sqrtf for x >= 2^-96; below that it stays under 2^-47,
+// beneath grad_spline_W's 1e-10 cut-off like the exact root (both checked
+// exhaustively on gfx950), and the gradient only compares it with the cut-off.
+__device__ __forceinline__ float sqrt_rn_pair(float x) {
+  float s = __builtin_amdgcn_sqrtf(x);
+  const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+  const float su = __uint_as_float(__float_as_uint(s) + 1u);
+  const float ed = __builtin_fmaf(-sd, s, x);
+  const float eu = __builtin_fmaf(-su, s, x);
+  s = ed <= 0.0f ? sd : s;
+  return eu > 0.0f ? su : s;
+}
+
+// grad of W at x_j - x_i = r, i.e. the table value (-W'/|r|) times r, for a
+// pair from the neighbour lists: |r| < h holds by construction
+// (the lists test the same float32 radicand with sqrtf(d2) < h), so only the
+// |r| < 1e-10 cut-off remains. The cut-off zeroes the table value, giving
+// (+-0, +-0, +-0) instead of (+0, +0, +0); every use adds or subtracts the
+// gradient (times other factors) into sums that start at +0, where a zero of
+// either sign leaves the bits of the sum unchanged.
+__device__ __forceinline__ float3 grad_w_pair(const float *G, int n, float scale, float3 r) {
+  const float len = sqrt_rn_pair(r.x * r.x + r.y * r.y + r.z * r.z);
   const float idx = len * scale;
   int lo = (int)idx;
   lo = lo < n - 2 ? lo : n - 2;
   const float a = idx - (float)lo;
-  const float s = (1.0f - a) * G[lo] + a * G[lo + 1];
-  const bool zero = len >= h || len < 1e-10f;
-  return make_float3(zero ? 0.0f : s * r.x, zero ? 0.0f : s * r.y, zero ? 0.0f : s * r.z);
+  float s = (1.0f - a) * G[lo] + a * G[lo + 1];
+  s = len < 1e-10f ? 0.0f : s;
+  return make_float3(s * r.x, s * r.y, s * r.z);
 }
 
 __device__ __forceinline__ float dist3(float ax, float ay, float az, float bx, float by, float bz) {
@@ -268,26 +288,41 @@ __device__ __forceinline__ int particle_index() { return xcd_tile() * blockDim.x
 
 // Visit the neighbours of particle i in list order. Lists are column-major,
 // entry k of particle i at list[k * n + i], so a wave reads slot k of 64
-// consecutive particles in one coalesced access. kBatch indices and their
-// gathers are issued before the batch is evaluated, so several memory round
-// trips are in flight per thread. `load(j, slot)` returns the data of
-// neighbour j (slot = its list position), `body(data)` accumulates in list
-// order.
+// consecutive particles in one coalesced access. The neighbours go in batches
+// of kBatch: the batch's gathers are issued together, then the list entries of
+// the next batch, and only then is the batch evaluated, so the list load of
+// one batch overlaps the gathers of the other. The last, partial batch repeats
+// entry cnt - 1 in its unused places (a valid index) and skips them in the
+// body. `load(j, slot)` returns the data of neighbour j (slot = its list
+// position), `body(data)` accumulates in list order.
 template <int kBatch, typename LoadF, typename BodyF>
 __device__ __forceinline__ void for_neighbors(const int *__restrict__ list, int n, int i, int cnt,
                                               LoadF load, BodyF body) {
+  if (cnt <= 0) return;
+  const int *__restrict__ col = list + i;
+  int jn[kBatch];
+#pragma unroll
+  for (int u = 0; u < kBatch; ++u) jn[u] = col[(long)min(u, cnt - 1) * n];
   int k = 0;
   for (; k + kBatch <= cnt; k += kBatch) {
-    int j[kBatch];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) j[u] = list[(long)(k + u) * n + i];
     decltype(load(0, 0L)) d[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) d[u] = load(j[u], (long)(k + u) * n + i);
+    for (int u = 0; u < kBatch; ++u) d[u] = load(jn[u], (long)(k + u) * n + i);
+    if (k + kBatch < cnt) {
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) jn[u] = col[(long)min(k + kBatch + u, cnt - 1) * n];
+    }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) body(d[u]);
   }
-  for (; k < cnt; ++k) body(load(list[(long)k * n + i], (long)k * n + i));
+  if (k < cnt) {
+    decltype(load(0, 0L)) d[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) d[u] = load(jn[u], (long)(k + u) * n + i);
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (k + u < cnt) body(d[u]);
+  }
 }
 
 // Fixed-point residual term of one particle (WSPH_RESIDUAL_* in wsph.h).

@@ -89,6 +89,7 @@ struct Bufs {
   // tables
   const float *wt, *gt;
   bool lds_tables;
+  int lds_per_cu;  // bytes of LDS per compute unit (host side: J2 occupancy cap)
   DevState *st;
 };
 

@@ -48,7 +48,8 @@ struct F4x2 {
   float4 a, b;
 };
 #define W_OF(T, r) spline_w((T).w, c.nw, c.w_scale, c.h, (r))
-#define GW_OF(T, r) grad_w((T).g, c.ng, c.g_scale, c.h, (r))
+// every GW_OF pair comes from the neighbour lists
+#define GW_OF(T, r) grad_w_pair((T).g, c.ng, c.g_scale, (r))
 
 // (dx*dx + dy*dy) + dz*dz with d = a - b: the radicand of wp.length(a - b)
 __device__ __forceinline__ float dist2(float ax, float ay, float az, float bx, float by, float bz) {
@@ -625,16 +626,18 @@ __device__ __forceinline__ void j1_particle(const Bufs &b, const SphConst &c, co
       [&](const F4x2 &v) {
         const float4 o = v.a, dj = v.b;
         const float3 gw = GW_OF(T, sub3(o, p));
-        // d_ji = grad W_ij * t3 / rho_i^2, d_ij = -grad W_ij * t3 / rho_j^2 (IISPH.py:372-373)
-        const float3 term = make_float3(div_by(gw.x * t3, ri2, yi) * pi - dj.x * dj.w,
-                                        div_by(gw.y * t3, ri2, yi) * pi - dj.y * dj.w,
-                                        div_by(gw.z * t3, ri2, yi) * pi - dj.z * dj.w);
+        // d_ji = grad W_ij * t3 / rho_i^2, d_ij = -grad W_ij * t3 / rho_j^2 (IISPH.py:372-373);
+        // both divide a = grad W_ij * t3, and (-a) / b = -(a / b) exactly
+        const float ax = gw.x * t3, ay = gw.y * t3, az = gw.z * t3;
+        const float3 term = make_float3(div_by(ax, ri2, yi) * pi - dj.x * dj.w,
+                                        div_by(ay, ri2, yi) * pi - dj.y * dj.w,
+                                        div_by(az, ri2, yi) * pi - dj.z * dj.w);
         t1 += dot3(term, gw);
         const float rj2 = o.w * o.w;
         const float yj = rcp_rn(rj2);
-        sx = sx + div_by(-gw.x * t3, rj2, yj) * dj.w;
-        sy = sy + div_by(-gw.y * t3, rj2, yj) * dj.w;
-        sz = sz + div_by(-gw.z * t3, rj2, yj) * dj.w;
+        sx = sx - div_by(ax, rj2, yj) * dj.w;
+        sy = sy - div_by(ay, rj2, yj) * dj.w;
+        sz = sz - div_by(az, rj2, yj) * dj.w;
       });
   b.sap[i] = make_float4(sx, sy, sz, t1 * c.mass);
 }
@@ -1360,6 +1363,18 @@ static inline size_t tab_bytes(const Bufs &b, const SphConst &c) {
   return b.lds_tables ? sizeof(float) * (size_t)(c.nw + c.ng) : 0;
 }
 
+// Dynamic LDS of a J2 launch: the tables, raised so that at most 4 waves per
+// SIMD (16 per CU) are resident. J2's neighbour gathers hit L1 more often with
+// fewer waves sharing it: measured at 1M particles, 512-thread blocks, 92 us
+// at 2 blocks per CU against 96 us at 3 (the register limit) and 115 us at 1.
+static inline size_t j2_lds_bytes(const Bufs &b, const SphConst &c, int block) {
+  const size_t tab = tab_bytes(b, c);
+  if (b.lds_per_cu <= 0) return tab;
+  const size_t cap = (size_t)b.lds_per_cu * (size_t)(block / 64) / 16;
+  const size_t want = cap > 1024 ? cap - 1024 : 0;  // below the share, whatever the granule
+  return tab > want ? tab : want;
+}
+
 #define DISPATCH_LDS(kern, grid, block, shm, stream, ...)                    \
   do {                                                                      \
     if (b.lds_tables)                                                       \
@@ -1379,15 +1394,21 @@ void launch_boundary_psi(const Bufs &b, const SphConst &c, hipStream_t s) {
 // bad[2]: pairs where div_by(a, b, rcp_rn(b)) != a / b, over every significand
 //   of b at 64 exponents in [-28, 35], each with a pseudo-random normal a of
 //   exponent in [-27, 27] (536,870,912 pairs; quotients are normal).
+// bad[3]: non-negative b up to +inf where sqrt_rn_pair(b) differs from sqrtf(b)
+//   (b >= 2^-96) or is not below 2^-47 (b < 2^-96).
 __global__ void k_selftest_arith(unsigned long long *bad) {
   const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
-  unsigned nr = 0, ns = 0, nd = 0;
+  unsigned nr = 0, ns = 0, nd = 0, np = 0;
   for (unsigned long long u = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
        u < (1ull << 32); u += stride) {
     const unsigned bits = (unsigned)u;
     const float b = __uint_as_float(bits);
     nr += __float_as_uint(rcp_rn(b)) != __float_as_uint(1.0f / b);
-    if (bits <= 0x7f800000u) ns += __float_as_uint(sqrt_rn(b)) != __float_as_uint(sqrtf(b));
+    if (bits <= 0x7f800000u) {
+      ns += __float_as_uint(sqrt_rn(b)) != __float_as_uint(sqrtf(b));
+      const float sp = sqrt_rn_pair(b);
+      np += bits >= 0x0f800000u ? __float_as_uint(sp) != __float_as_uint(sqrtf(b)) : !(sp < 0x1.0p-47f);
+    }
     if (bits < (1u << 29)) {
       const unsigned e = 99u + (bits >> 23);  // 64 exponents
       const float bb = __uint_as_float((e << 23) | (bits & 0x7fffffu));
@@ -1399,11 +1420,13 @@ __global__ void k_selftest_arith(unsigned long long *bad) {
       nd += __float_as_uint(div_by(a, bb, rcp_rn(bb))) != __float_as_uint(a / bb);
     }
   }
-  const unsigned long long r = wave_sum_u64(nr), q = wave_sum_u64(ns), d = wave_sum_u64(nd);
+  const unsigned long long r = wave_sum_u64(nr), q = wave_sum_u64(ns), d = wave_sum_u64(nd),
+                           e = wave_sum_u64(np);
   if ((threadIdx.x & 63) == 0) {
     if (r) atomicAdd(&bad[0], r);
     if (q) atomicAdd(&bad[1], q);
     if (d) atomicAdd(&bad[2], d);
+    if (e) atomicAdd(&bad[3], e);
   }
 }
 
@@ -1443,7 +1466,7 @@ void launch_j1(const Bufs &b, const SphConst &c, int src, int block, hipStream_t
 
 void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, hipStream_t s) {
   if (b.n == 0) return;
-  DISPATCH_LDS(k_j2, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src, slot);
+  DISPATCH_LDS(k_j2, nblocks(b.n, block), block, j2_lds_bytes(b, c, block), s, b, c, src, slot);
   if (slot >= 0)
     k_residual_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block), slot, 0, 0, 0, 0, 0, 0.0);
 }
@@ -1456,7 +1479,7 @@ void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P,
     if (ev) (void)hipEventRecord(ev3[0], s);
     DISPATCH_LDS(k_j1_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0);
     if (ev) (void)hipEventRecord(ev3[1], s);
-    DISPATCH_LDS(k_j2_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0);
+    DISPATCH_LDS(k_j2_dev, nblocks(b.n, block), block, j2_lds_bytes(b, c, block), s, b, c, src0);
     if (ev) (void)hipEventRecord(ev3[2], s);
     if (b.scenes)
       k_batch_stop<<<1, 1024, 0, s>>>(b, src0, P.min_iters, P.max_iters, P.force_iters, P.eta);
