@@ -194,7 +194,8 @@ int wsph_set_tables(wsph_solver *s, const float *w_table, int w_size,
 
 /* Upload boundary particles (device float[nb][3] each; vb may be NULL for
  * zero velocity), build the boundary hash grid and compute psi
- * (IISPH.py:811-821 -> compute_boundary_density, IISPH.py:23-47). */
+ * (IISPH.py:811-821 -> compute_boundary_density, IISPH.py:23-47).
+ * At most 2^28 - 1 particles (WSPH_E_CAPACITY), here and in wsph_set_fluid. */
 int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb);
 
 /* Rigid motion of the boundary particles with original index >= first (the
