@@ -92,6 +92,20 @@ def test_neighbour_capacity_overflow_is_an_error():
     assert s2.step_stats()["max_ff_count"] <= cfg2.max_ff
 
 
+def test_particle_count_limit_is_an_error():
+    """More than 2^28 - 1 particles (the gathers' 32-bit byte offsets) is a
+    capacity error, raised before the pointer is read."""
+    _announce("edge: particle count limit")
+    cfg = config.dambreak_8k()
+    s = _solver(cfg, scene.build(cfg))
+    x = s.x
+    for fn in ("wsph_set_fluid", "wsph_set_boundary"):
+        rc = getattr(s._lib, fn)(s._h, x.data_ptr(), None, 1 << 28)
+        assert rc == N.WSPH_E_CAPACITY, (fn, rc)
+        assert b"at most" in s._lib.wsph_last_error()
+    s.step()  # the solver is unchanged
+
+
 def test_batch_with_an_empty_scene():
     from warpsph_amd.batch import BatchIISPH
     from warpsph_amd.solver import IISPH

@@ -16,6 +16,8 @@ PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "libwsph.so"
 
 WSPH_OK = 0
+WSPH_E_ARG, WSPH_E_HIP, WSPH_E_CAPACITY, WSPH_E_NAN, WSPH_E_BARRIER, WSPH_E_UNSUPPORTED = \
+    -1, -2, -3, -4, -5, -6
 F_X, F_V, F_P, F_RHO, F_RHO_ADV, F_V_ADV, F_D_II, F_A_II, F_AP, F_S, \
     F_FF_COUNT, F_FS_COUNT, F_PSI, F_CELL_KEY, F_BX, F_BV = range(16)
 FIELD_WIDTH = {F_X: 3, F_V: 3, F_P: 1, F_RHO: 1, F_RHO_ADV: 1, F_V_ADV: 3,

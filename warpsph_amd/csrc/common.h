@@ -222,8 +222,9 @@ __device__ __forceinline__ float sqrt_rn_pair(float x) {
 // (+-0, +-0, +-0) instead of (+0, +0, +0); every use adds or subtracts the
 // gradient (times other factors) into sums that start at +0, where a zero of
 // either sign leaves the bits of the sum unchanged.
-__device__ __forceinline__ float3 grad_w_pair(const float *G, int n, float scale, float3 r) {
-  const float len = sqrt_rn_pair(r.x * r.x + r.y * r.y + r.z * r.z);
+// grad_w_pair given len, a square root of |r|^2 that is exact from 2^-96 up and
+// below 1e-10 under it (sqrt_rn or sqrt_rn_pair).
+__device__ __forceinline__ float3 grad_w_len(const float *G, int n, float scale, float len, float3 r) {
   const float idx = len * scale;
   int lo = (int)idx;
   lo = lo < n - 2 ? lo : n - 2;
@@ -231,6 +232,9 @@ __device__ __forceinline__ float3 grad_w_pair(const float *G, int n, float scale
   float s = (1.0f - a) * G[lo] + a * G[lo + 1];
   s = len < 1e-10f ? 0.0f : s;
   return make_float3(s * r.x, s * r.y, s * r.z);
+}
+__device__ __forceinline__ float3 grad_w_pair(const float *G, int n, float scale, float3 r) {
+  return grad_w_len(G, n, scale, sqrt_rn_pair(r.x * r.x + r.y * r.y + r.z * r.z), r);
 }
 
 __device__ __forceinline__ float dist3(float ax, float ay, float az, float bx, float by, float bz) {

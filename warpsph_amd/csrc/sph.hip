@@ -433,6 +433,11 @@ __device__ __forceinline__ float4 vadv_plain(float4 v, float rho, float dt, cons
 }
 
 // ---- compute_density, IISPH.py:177-214 -----------------------------------
+// Also the gradient sums of d_ii (compute_term_d, IISPH.py:359-382), which need
+// only positions, psi and rho_i: m sum_j -grad W_ij + sum_b -psi_b grad W_ib
+// times dt^2 / rho_i^2, accumulated in list order in the same loops and left
+// in sap for k_assemble. One rounded square root serves W and grad W: the
+// radicand of wp.length(x_i - x_j) and of |x_j - x_i| is the same float.
 template <bool kLds>
 __global__ void k_density(Bufs b, SphConst c, int fuse_vadv) {
   const Tabs T = stage_tabs<kLds>(b, c);
@@ -443,15 +448,37 @@ __global__ void k_density(Bufs b, SphConst c, int fuse_vadv) {
   const float4 *__restrict__ pos = b.pos;
   const float4 *__restrict__ bpos = b.bpos;
   float t1 = 0.0f, t2 = 0.0f;
+  float dx1 = 0.0f, dy1 = 0.0f, dz1 = 0.0f, dx2 = 0.0f, dy2 = 0.0f, dz2 = 0.0f;
   for_neighbors<kBatch>(
       b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return ld_at(pos, j); },
-      [&](const float4 &o) { t1 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z)); });
+      [&](const float4 &o) {
+        const float3 r = sub3(o, p);
+        const float len = sqrt_rn(r.x * r.x + r.y * r.y + r.z * r.z);
+        t1 += W_OF(T, len);
+        const float3 gw = grad_w_len(T.g, c.ng, c.g_scale, len, r);
+        dx1 = dx1 - gw.x;
+        dy1 = dy1 - gw.y;
+        dz1 = dz1 - gw.z;
+      });
   for_neighbors<kBatch>(
       b.nbr_fs, b.n, i, b.cnt_fs[i], [&](int j, long) { return ld_at(bpos, j); },
-      [&](const float4 &o) { t2 += W_OF(T, dist3(p.x, p.y, p.z, o.x, o.y, o.z)) * o.w; });
+      [&](const float4 &o) {
+        const float3 r = sub3(o, p);
+        const float len = sqrt_rn(r.x * r.x + r.y * r.y + r.z * r.z);
+        t2 += W_OF(T, len) * o.w;
+        const float3 gw = grad_w_len(T.g, c.ng, c.g_scale, len, r);
+        dx2 = dx2 - gw.x * o.w;
+        dy2 = dy2 - gw.y * o.w;
+        dz2 = dz2 - gw.z * o.w;
+      });
   const float rho = t1 * c.mass + t2 + W_OF(T, 0.0f) * c.mass;
   reinterpret_cast<float *>(&b.pos[i])[3] = rho;
-  if (fuse_vadv) b.vadv[i] = vadv_plain(b.vel[i], rho, (float)b.st->dt, c);
+  const float dt = (float)b.st->dt;
+  const float dt2 = dt * dt;
+  const float ri2 = rho * rho;
+  b.sap[i] = make_float4((dx1 * c.mass + dx2) * dt2 / ri2, (dy1 * c.mass + dy2) * dt2 / ri2,
+                         (dz1 * c.mass + dz2) * dt2 / ri2, 0.0f);
+  if (fuse_vadv) b.vadv[i] = vadv_plain(b.vel[i], rho, dt, c);
 }
 
 // ---- predict_v_adv, IISPH.py:217-278 (+ XSPH smoothing) ------------------
@@ -546,16 +573,20 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
   const float4 *__restrict__ vadv = b.vadv;
   const float4 *__restrict__ bpos = b.bpos;
   const float4 *__restrict__ bvel = b.bvel;
-  float tr1 = 0.0f, tr2 = 0.0f;
-  float dx1 = 0.0f, dy1 = 0.0f, dz1 = 0.0f, dx2 = 0.0f, dy2 = 0.0f, dz2 = 0.0f;
+  // d_ii from k_density; rho_adv (predict_rho_adv) and a_ii (compute_term_a)
+  // share the neighbour loops, each sum in list order
+  const float4 di = b.sap[i];
+  const float dix = di.x, diy = di.y, diz = di.z;
+  float tr1 = 0.0f, tr2 = 0.0f, ta1 = 0.0f, ta2 = 0.0f;
   for_neighbors<kBatch>(
       b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{ld_at(pos, j), ld_at(vadv, j)}; },
       [&](const F4x2 &v) {
         const float3 gw = GW_OF(T, sub3(v.a, p));
         tr1 += dot3(make_float3(va.x - v.b.x, va.y - v.b.y, va.z - v.b.z), gw);
-        dx1 = dx1 - gw.x;
-        dy1 = dy1 - gw.y;
-        dz1 = dz1 - gw.z;
+        const float3 dd = make_float3(dix - div_by(gw.x * t3, ri2, yi),
+                                      diy - div_by(gw.y * t3, ri2, yi),
+                                      diz - div_by(gw.z * t3, ri2, yi));
+        ta1 += dot3(dd, gw);
       });
   for_neighbors<kBatch>(
       b.nbr_fs, b.n, i, ns, [&](int j, long) { return F4x2{ld_at(bpos, j), ld_at(bvel, j)}; },
@@ -563,32 +594,9 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
         const float4 o = v.a, vb = v.b;
         const float3 gw = GW_OF(T, sub3(o, p));
         tr2 += dot3(make_float3(va.x - vb.x, va.y - vb.y, va.z - vb.z), gw) * o.w;
-        dx2 = dx2 - gw.x * o.w;
-        dy2 = dy2 - gw.y * o.w;
-        dz2 = dz2 - gw.z * o.w;
+        ta2 += dot3(make_float3(dix, diy, diz), gw) * o.w;
       });
   const float rho_adv = p.w + (tr1 * c.mass + tr2) * dt;
-  const float dix = (dx1 * c.mass + dx2) * dt2 / ri2;
-  const float diy = (dy1 * c.mass + dy2) * dt2 / ri2;
-  const float diz = (dz1 * c.mass + dz2) * dt2 / ri2;
-  float ta1 = 0.0f, ta2 = 0.0f;
-  for_neighbors<kBatch>(
-      b.nbr_ff, b.n, i, nf, [&](int j, long) { return ld_at(pos, j); },
-      [&](const float4 &va_) {
-        const F4x2 v{va_, va_};
-        const float3 gw = GW_OF(T, sub3(v.a, p));
-        const float3 dd = make_float3(dix - div_by(gw.x * t3, ri2, yi),
-                                      diy - div_by(gw.y * t3, ri2, yi),
-                                      diz - div_by(gw.z * t3, ri2, yi));
-        ta1 += dot3(dd, gw);
-      });
-  for_neighbors<kBatch>(
-      b.nbr_fs, b.n, i, ns, [&](int j, long) { return ld_at(bpos, j); },
-      [&](const float4 &va_) {
-        const F4x2 v{va_, va_};
-        const float3 gw = GW_OF(T, sub3(v.a, p));
-        ta2 += dot3(make_float3(dix, diy, diz), gw) * v.a.w;
-      });
   const float a_ii = ta1 * c.mass + ta2;
   b.aux[i] = make_float4(rho_adv, a_ii, 0.0f, 0.0f);
   float pv;
