@@ -241,40 +241,6 @@ __device__ __forceinline__ void for_query_rows(float px, float py, float pz, boo
   }
 }
 
-// Candidates [s0, e0) of one span tested against p without branches, 32 at a
-// time, into a per-lane hit mask; `put` then runs once per hit, in ascending
-// candidate order (the order of a one-by-one scan). self: the index to skip
-// (the particle itself), -1 for none.
-template <typename PutF>
-__device__ __forceinline__ void span_hits(const float4 *__restrict__ pts, int s0, int e0,
-                                          const float4 &p, float r2_lt, int self, PutF put) {
-  for (int base = s0; base < e0; base += 32) {
-    const int cnt = e0 - base < 32 ? e0 - base : 32;
-    unsigned m = 0u;
-    int u = 0;
-    for (; u + kBatch <= cnt; u += kBatch) {
-      float4 o[kBatch];
-#pragma unroll
-      for (int k = 0; k < kBatch; ++k) o[k] = pts[base + u + k];
-#pragma unroll
-      for (int k = 0; k < kBatch; ++k) {
-        const bool hit = (base + u + k != self) & (dist2(p.x, p.y, p.z, o[k].x, o[k].y, o[k].z) < r2_lt);
-        m |= (unsigned)hit << (u + k);
-      }
-    }
-    for (; u < cnt; ++u) {
-      const float4 o = pts[base + u];
-      const bool hit = (base + u != self) & (dist2(p.x, p.y, p.z, o.x, o.y, o.z) < r2_lt);
-      m |= (unsigned)hit << u;
-    }
-    while (m) {
-      const int bit = __builtin_ctz(m);
-      m &= m - 1u;
-      put(base + bit);
-    }
-  }
-}
-
 // Neighbour list writer of one wave. Entry k of lane l belongs at
 // list[k * n + i_l]; the lanes find their entries at different times, and
 // storing each as found fills a 128-byte line of slot k from ~32 separate
@@ -284,7 +250,7 @@ __device__ __forceinline__ void span_hits(const float4 *__restrict__ pts, int s0
 // coalesced row, when every lane of the wave has passed it (after each query
 // row) or at the end. A lane more than kListRing slots ahead of the wave's
 // flushed rows stores its later entries directly.
-constexpr int kListRing = 16;  // a power of two
+constexpr int kListRing = 32;  // a power of two
 static_assert((kListRing & (kListRing - 1)) == 0, "ring rows: a power of two");
 struct ListWriter {
   int *ring;      // [kListRing][64] of this wave
@@ -304,6 +270,18 @@ struct ListWriter {
     }
     ++cnt;
   }
+  // Room for k more entries in the ring without reaching an unflushed row or
+  // the capacity: then put_fast may be used k times.
+  __device__ bool room(int k) const {
+    return direct == 0x7fffffff && cnt + k - flushed <= kListRing && cnt + k <= cap;
+  }
+  // Candidate j with its test result, without a branch: j goes to the next
+  // free ring slot, which only a hit keeps (a miss is overwritten by the next
+  // candidate, and slots at or past cnt are never flushed).
+  __device__ void put_fast(int j, bool hit) {
+    ring[((unsigned)cnt & (kListRing - 1)) * 64 + lane] = j;
+    cnt += hit ? 1 : 0;
+  }
   __device__ void flush_to(int upto) {  // wave-uniform
     for (int k = flushed; k < upto; ++k)
       if (live && k < cnt && k < cap && k < direct) list[(long)k * n + i] = ring[((unsigned)k & (kListRing - 1)) * 64 + lane];
@@ -320,6 +298,49 @@ struct ListWriter {
     flush_to(m);
   }
 };
+
+// Candidates [s0, e0) of one span tested against p in batches of kBatch
+// loads, each result written with put_fast (no branch per candidate) while
+// the ring has room, else with put for the hits; the last, partial batch
+// loads candidate e0 - 1 in its unused places and masks them out. Hits go to
+// the writer in ascending candidate order (the order of a one-by-one scan).
+// self: the index to skip (the particle itself), -1 for none.
+__device__ __forceinline__ void emit_batch(ListWriter &w, int j, const bool (&hit)[kBatch]) {
+  if (w.room(kBatch)) {
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) w.put_fast(j + u, hit[u]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (hit[u]) w.put(j + u);
+  }
+}
+
+__device__ __forceinline__ void span_hits(const float4 *__restrict__ pts, int s0, int e0,
+                                          const float4 &p, float r2_lt, int self, ListWriter &w) {
+  int j = s0;
+  for (; j + kBatch <= e0; j += kBatch) {
+    float4 o[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) o[u] = pts[j + u];
+    bool hit[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      hit[u] = (j + u != self) & (dist2(p.x, p.y, p.z, o[u].x, o[u].y, o[u].z) < r2_lt);
+    emit_batch(w, j, hit);
+  }
+  if (j < e0) {
+    float4 o[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) o[u] = pts[min(j + u, e0 - 1)];
+    bool hit[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      hit[u] = (j + u < e0) & (j + u != self) &
+               (dist2(p.x, p.y, p.z, o[u].x, o[u].y, o[u].z) < r2_lt);
+    emit_batch(w, j, hit);
+  }
+}
 
 // Whether particle p can have boundary candidates: its query range lies within
 // 2 cells of its own cell on every axis, and the slots within 2 cells of every
@@ -351,7 +372,7 @@ __global__ void k_lists(Bufs b, SphConst c) {
       p.x, p.y, p.z, live, c, fg.dims, fg.start, fg.count,
       [&](int s0, int e0) {
         // distance < SMOOTHING_LENGTH and index != i (IISPH.py:72)
-        span_hits(b.pos, s0, e0, p, c.r2_lt, i, [&](int j) { wf.put(j); });
+        span_hits(b.pos, s0, e0, p, c.r2_lt, i, wf);
       },
       [&]() { wf.row_done(); });
   wf.finish();
@@ -362,7 +383,7 @@ __global__ void k_lists(Bufs b, SphConst c) {
         p.x, p.y, p.z, live, c, bg.dims, bg.start, bg.count,
         [&](int s0, int e0) {
           // distance < SMOOTHING_LENGTH (IISPH.py:129)
-          span_hits(b.bpos, s0, e0, p, c.r2_lt, -1, [&](int j) { wb.put(j); });
+          span_hits(b.bpos, s0, e0, p, c.r2_lt, -1, wb);
         },
         [&]() { wb.row_done(); });
     wb.finish();
