@@ -151,6 +151,26 @@ def cfg_n(x):
     return int(x.shape[0])
 
 
+PHASE_STEPS = 40
+
+
+def phase_split(lib, h, steps: int = PHASE_STEPS) -> dict:
+    """ms per step of the four phases, from ``steps`` further steps with phase
+    events (timing mode 1). The timed run itself records only the J1/J2
+    events of every 4th step (mode 2): an event between two kernels costs a
+    few microseconds of device time, which phase events add to every step."""
+    from warpsph_amd import _native as N
+    N.check("wsph_set_timing", lib.wsph_set_timing(h, 1))
+    for _ in range(steps):
+        N.check("wsph_step", lib.wsph_step(h, None))
+    tm = (C.c_double * 10)()
+    N.check("wsph_last_timings", lib.wsph_last_timings(h, tm, 10))
+    N.check("wsph_set_timing", lib.wsph_set_timing(h, 0))
+    k = max(1, int(tm[6]))
+    return {"neighbor_search": tm[0] / k, "predict_advection": tm[1] / k,
+            "pressure_solve": tm[2] / k, "integrate": tm[3] / k, "steps": int(tm[6])}
+
+
 def timed_run(cfg, net, args, trace: bool) -> dict:
     """W untimed steps, then K steps timed between device synchronisations."""
     import torch
@@ -175,7 +195,7 @@ def timed_run(cfg, net, args, trace: bool) -> dict:
         N.check("wsph_step", lib.wsph_step(h, None))
     N.check("wsph_sync", lib.wsph_sync(h))
     c0 = s.counters()
-    N.check("wsph_set_timing", lib.wsph_set_timing(h, 1))
+    N.check("wsph_set_timing", lib.wsph_set_timing(h, 2))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -191,7 +211,10 @@ def timed_run(cfg, net, args, trace: bool) -> dict:
     steps = c1["steps"] - c0["steps"]
     nonconv = c1["nonconverged"] - c0["nonconverged"]
     elapsed = t1 - t0
-    return dict(solver=s, n=s.n, elapsed=elapsed, steps=steps, tm=tm,
+    h_rec = s.history(args.steps) if net is not None else None
+    phases = phase_split(lib, h) if trace else None
+    return dict(solver=s, n=s.n, elapsed=elapsed, steps=steps, tm=tm, phases=phases,
+                history=h_rec,
                 iters_per_step=(c1["iterations"] - c0["iterations"]) / max(1, steps),
                 nonconverged=nonconv, ap_evals=c1["ap_evals"] - c0["ap_evals"],
                 value=s.n * (steps - nonconv) / elapsed)
@@ -231,7 +254,7 @@ def bench_batch(args) -> None:
         N.check("wsph_step", lib.wsph_step(h, None))
     N.check("wsph_sync", lib.wsph_sync(h))
     c0 = b.counters()
-    N.check("wsph_set_timing", lib.wsph_set_timing(h, 1))
+    N.check("wsph_set_timing", lib.wsph_set_timing(h, 2))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -248,7 +271,6 @@ def bench_batch(args) -> None:
     nonconv = c1["nonconverged"] - c0["nonconverged"]
     last = b.scene_stats()
     n = b.n
-    jac_ms = tm[2] / max(1, int(tm[6]))
     ksteps = max(1, int(tm[9]))
     pair_ms = (tm[7] + tm[8]) / ksteps
     pair_bytes = n * BYTES_PER_AP_EVAL
@@ -268,9 +290,7 @@ def bench_batch(args) -> None:
         "last_step_scene_iterations": {"min": int(last["iterations"].min()),
                                        "median": float(np.median(last["iterations"])),
                                        "max": int(last["iterations"].max())},
-        "phase_ms_per_step": {"neighbor_search": tm[0] / max(1, tm[6]),
-                              "predict_advection": tm[1] / max(1, tm[6]),
-                              "pressure_solve": jac_ms, "integrate": tm[3] / max(1, tm[6])},
+        "phase_ms_per_step": phase_split(lib, h),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_j1_dev + k_j2_dev (one A p evaluation, first iteration)",
@@ -436,8 +456,7 @@ def main():
     r = timed_run(cfg, net, args, trace=True)
     s, n, elapsed, steps, tm = r["solver"], r["n"], r["elapsed"], r["steps"], r["tm"]
     ap_evals = r["ap_evals"]
-    jac_ms = tm[2] / max(1, int(tm[6]))
-    ktm, kernel_source = tm, "the timed run"
+    ktm, kernel_source = tm, "the timed run (every 4th step)"
     if int(tm[9]) == 0:
         # graph replay: the first iteration's J1 and J2 run inside the captured
         # graph, where no events are recorded; time the same kernels in a short
@@ -473,10 +492,7 @@ def main():
                    "warm_start": args.warm_start, "eta": cfg.eta},
         "jacobi_iters_per_step": r["iters_per_step"],
         "nonconverged_steps": r["nonconverged"],
-        "phase_ms_per_step": {"neighbor_search": tm[0] / max(1, tm[6]),
-                              "predict_advection": tm[1] / max(1, tm[6]),
-                              "pressure_solve": jac_ms,
-                              "integrate": tm[3] / max(1, tm[6])},
+        "phase_ms_per_step": r["phases"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(workload),
@@ -494,7 +510,7 @@ def main():
         line["autotune"] = {"cached": tuned["cached"], "ms_per_step": tuned["ms_per_step"],
                             "candidates": len(tuned.get("table", []))}
     if net is not None:
-        h = s.history(args.steps)
+        h = r["history"]
         line["learned"] = {
             "steps_on_network": sum(1 for q in h if q["start_source"] == config.WARM_LEARNED),
             "start_err_learned_pa": sum(q["start_err_learned"] for q in h) / max(1, len(h)),

@@ -60,6 +60,30 @@ def test_graph_step_zero_start_and_timing():
     assert tm[6] == 10 and tm[4] > 0
 
 
+def test_kernel_timing_mode():
+    """Timing mode 2 records only the first iteration's J1/J2 events, on
+    every 4th step, and leaves the phase sums at zero; the device-driven loop
+    paces itself on the pinned progress word, with the same results as an
+    untimed solver."""
+    from warpsph_amd import _native as N
+    import ctypes as C
+    cfg = config.dambreak_8k()
+    _announce("timing mode 2 (kernel events only)")
+    a, b = _pair(cfg, 0)
+    with pytest.raises(RuntimeError, match="timing mode"):
+        N.check("wsph_set_timing", b._lib.wsph_set_timing(b._h, 3))
+    N.check("wsph_set_timing", b._lib.wsph_set_timing(b._h, 2))
+    for _ in range(12):
+        a.step()
+        b.step()
+    assert_same("x", b.x.cpu().numpy(), a.x.cpu().numpy())
+    assert b.counters() == a.counters()
+    tm = (C.c_double * 10)()
+    N.check("wsph_last_timings", b._lib.wsph_last_timings(b._h, tm, 10))
+    assert tm[6] == 12 and tm[4] == 0 and tm[0] == 0
+    assert tm[9] == 3 and tm[7] > 0 and tm[8] > 0
+
+
 def test_graph_step_learned_start():
     from warpsph_amd.mlp import PressureNet
     cfg = config.dambreak_8k().replace(jitter=0.01, warm_start=config.WARM_LEARNED)

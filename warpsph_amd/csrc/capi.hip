@@ -205,10 +205,11 @@ struct wsph_solver {
   bool searched = false, advected = false, solved = false;
   // timing: a ring of per-step event sets, read back lazily so that timed
   // steps need no host synchronisation
-  bool timing = false;
+  int timing = 0;  // 0 off, 1 phases and kernels, 2 kernels of every kKernelEvery-th step
   // per step: 5 phase boundaries, then 3 events around J1 and J2 of the
   // first Jacobi iteration (device driver; that iteration always does work)
   static constexpr int kRing = 64, kEv = 8;
+  static constexpr int kKernelEvery = 4;  // timing mode 2: J1/J2 events on every 4th step
   hipEvent_t ev[kRing * kEv] = {};
   bool kev_rec[kRing] = {};
   double k_sum[2] = {0, 0};
@@ -219,7 +220,7 @@ struct wsph_solver {
   // device-driven Jacobi loop: pinned host copy of DevState::done_tag, stored
   // by the device's stop test
   int *done_host = nullptr;
-  hipEvent_t chunk_ev[2] = {};
+  unsigned long long *prog_host = nullptr;
   int step_count = 0;
   // captured step graphs, one per fluid buffer parity at step start
   hipGraph_t graph[2] = {nullptr, nullptr};
@@ -339,6 +340,7 @@ static Bufs make_bufs(wsph_solver *s) {
   }
   b.err_part = s->err_part;
   b.done_host = s->done_host;
+  b.prog_host = s->prog_host;
   b.arbiter = learned && s->net && s->err_part;
   if (s->nscenes > 0) {
     b.scenes = s->d_scenes;
@@ -466,9 +468,12 @@ int wsph_create(const wsph_params *params, wsph_solver **out) {
   CREATE_TRY(hipMemsetAsync(s->totals, 0, sizeof(long long) * 8, s->stream));
   CREATE_TRY(hipMemsetAsync(s->st, 0, sizeof(DevState), s->stream));
   for (auto &e : s->ev) CREATE_TRY(hipEventCreate(&e));
-  for (auto &e : s->chunk_ev) CREATE_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  CREATE_TRY(hipHostMalloc((void **)&s->done_host, sizeof(int), hipHostMallocDefault));
+  // coherent (fine-grained) pinned words: the device's stores reach the host
+  // while its kernels run, without a release at a kernel or event boundary
+  CREATE_TRY(hipHostMalloc((void **)&s->done_host, sizeof(int), hipHostMallocCoherent));
   *s->done_host = -1;
+  CREATE_TRY(hipHostMalloc((void **)&s->prog_host, sizeof(unsigned long long), hipHostMallocCoherent));
+  *s->prog_host = ~0ull;
   k_set_dt<<<1, 1, 0, s->stream>>>(s->st, P.dt, 0.0);
   CREATE_TRY(hipGetLastError());
   int rcg = alloc_grid(s->fg, P.grid_dims, P.cell_layout);
@@ -589,9 +594,8 @@ int wsph_destroy(wsph_solver *s) {
   }
   for (auto &e : s->ev)
     if (e) (void)hipEventDestroy(e);
-  for (auto &e : s->chunk_ev)
-    if (e) (void)hipEventDestroy(e);
   if (s->done_host) (void)hipHostFree(s->done_host);
+  if (s->prog_host) (void)hipHostFree(s->prog_host);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return WSPH_OK;
@@ -794,7 +798,7 @@ int wsph_set_pressure(wsph_solver *s, const float *p) {
 }
 
 static void rec(wsph_solver *s, int k) {
-  if (s->timing)
+  if (s->timing == 1)
     (void)hipEventRecord(s->ev[(s->t_recorded % wsph_solver::kRing) * wsph_solver::kEv + k],
                          s->stream);
 }
@@ -802,14 +806,18 @@ static void rec(wsph_solver *s, int k) {
 // Fold the oldest recorded step into the phase sums.
 static void timing_fold_one(wsph_solver *s) {
   const int base = (int)(s->t_processed % wsph_solver::kRing) * wsph_solver::kEv;
-  (void)hipEventSynchronize(s->ev[base + 4]);
-  float ms;
-  for (int k = 0; k < 4; ++k) {
-    if (hipEventElapsedTime(&ms, s->ev[base + k], s->ev[base + k + 1]) == hipSuccess)
-      s->t_sum[k] += ms;
-  }
-  if (hipEventElapsedTime(&ms, s->ev[base], s->ev[base + 4]) == hipSuccess) s->t_sum[4] += ms;
   const int slot = (int)(s->t_processed % wsph_solver::kRing);
+  float ms;
+  if (s->timing == 1) {
+    (void)hipEventSynchronize(s->ev[base + 4]);
+    for (int k = 0; k < 4; ++k) {
+      if (hipEventElapsedTime(&ms, s->ev[base + k], s->ev[base + k + 1]) == hipSuccess)
+        s->t_sum[k] += ms;
+    }
+    if (hipEventElapsedTime(&ms, s->ev[base], s->ev[base + 4]) == hipSuccess) s->t_sum[4] += ms;
+  } else if (s->kev_rec[slot]) {
+    (void)hipEventSynchronize(s->ev[base + 7]);
+  }
   if (s->kev_rec[slot]) {
     float m1, m2;
     if (hipEventElapsedTime(&m1, s->ev[base + 5], s->ev[base + 6]) == hipSuccess &&
@@ -826,7 +834,7 @@ static void timing_fold_one(wsph_solver *s) {
 // Events around the first Jacobi iteration's J1 and J2 of the step being
 // recorded (null when timing is off).
 static void kernel_events(wsph_solver *s, hipEvent_t ev3[3]) {
-  if (!s->timing) {
+  if (!s->timing || (s->timing == 2 && s->t_recorded % wsph_solver::kKernelEvery != 0)) {
     ev3[0] = ev3[1] = ev3[2] = nullptr;
     return;
   }
@@ -1086,12 +1094,33 @@ int wsph_predict_advection(wsph_solver *s) {
   return WSPH_OK;
 }
 
+// Wait until the device has run the stop test of the first `iters` Jacobi
+// iterations of step `tag` (the progress word the stop kernel stores in
+// pinned memory) or ended the step's loop (done_host == tag): 1 when it
+// ended, 0 when it has only progressed, < 0 on a stream error. The host
+// spins on the two words; every 1024 polls it also asks the stream, which,
+// once idle, has run everything enqueued. No event is recorded: an event
+// between two kernels costs ~6 us of device time on this stream.
+static int wait_progress(wsph_solver *s, int tag, int iters) {
+  const unsigned long long want_tag = (unsigned)tag;
+  for (long spin = 1;; ++spin) {
+    if (__atomic_load_n(s->done_host, __ATOMIC_ACQUIRE) == tag) return 1;
+    const unsigned long long p = __atomic_load_n(s->prog_host, __ATOMIC_ACQUIRE);
+    if ((p >> 32) == want_tag && (p & 0xffffffffull) >= (unsigned)iters) return 0;
+    if ((spin & 1023) == 0) {
+      const hipError_t e = hipStreamQuery(s->stream);
+      if (e == hipSuccess) return __atomic_load_n(s->done_host, __ATOMIC_ACQUIRE) == tag ? 1 : 0;
+      if (e != hipErrorNotReady) return fail(WSPH_E_HIP, "hipStreamQuery: %s", hipGetErrorString(e));
+    }
+    __builtin_ia32_pause();
+  }
+}
+
 // Device-driven Jacobi loop: iterations queued in chunks, one chunk ahead of
-// the host's look at the device's stop flag. `queued` iterations in m0 chunks
-// were enqueued already. The stop test stores the step's tag in the pinned
-// done_host when the loop ends; once a chunk's event has completed, everything
-// that chunk stored is visible to the host (a later chunk can only store the
-// same tag).
+// the host's look at the device's progress. `queued` iterations in m0 chunks
+// were enqueued already. Before enqueuing chunk m + 1 the host waits for the
+// stop tests of chunk m - 1, and it stops enqueuing once the device reports
+// the loop ended (chunks already queued return at once).
 static int device_loop(wsph_solver *s, const Bufs &b, int src, int queued, int m0) {
   const wsph_params &P = s->P;
   const int C = P.chunk_iters > 0 ? P.chunk_iters : 2;
@@ -1102,12 +1131,13 @@ static int device_loop(wsph_solver *s, const Bufs &b, int src, int queued, int m
     hipEvent_t ev3[3] = {nullptr, nullptr, nullptr};
     if (queued == 0) kernel_events(s, ev3);
     launch_jacobi_chunk(b, s->C, P, src, cnt, P.block, s->stream, ev3);
+    const int before = queued;  // iterations of the chunks before this one
     queued += cnt;
-    HIP_TRY(hipEventRecord(s->chunk_ev[m & 1], s->stream));
     if (queued >= max_it) break;
     if (m >= 1) {
-      HIP_TRY(hipEventSynchronize(s->chunk_ev[(m - 1) & 1]));
-      if (__atomic_load_n(s->done_host, __ATOMIC_ACQUIRE) == tag) break;
+      const int rc = wait_progress(s, tag, before);
+      if (rc < 0) return rc;
+      if (rc == 1) break;
     }
   }
   launch_batch_finalize(b, s->stream);
@@ -1330,7 +1360,6 @@ static int graph_step(wsph_solver *s, wsph_step_stats *stats) {
   s->step_count++;
   rec(s, 0);
   HIP_TRY(hipGraphLaunch(s->gexec[c], s->stream));
-  HIP_TRY(hipEventRecord(s->chunk_ev[0], s->stream));
   s->cur = c ^ 1;
   s->searched = s->advected = true;
   rec(s, 1);
@@ -1646,7 +1675,8 @@ int wsph_hash_keys(const float *x, int n, float cell_width, const int dims[3], i
 int wsph_set_timing(wsph_solver *s, int enable) {
   if (!s) return fail(WSPH_E_ARG, "null solver");
   HIP_TRY(hipStreamSynchronize(s->stream));
-  s->timing = enable != 0;
+  if (enable < 0 || enable > 2) return fail(WSPH_E_ARG, "timing mode must be 0, 1 or 2");
+  s->timing = enable;
   s->t_recorded = s->t_processed = 0;
   for (double &v : s->t_sum) v = 0.0;
   s->k_sum[0] = s->k_sum[1] = 0.0;

@@ -86,6 +86,7 @@ struct Bufs {
   // cells on every axis (null: not known, every query searches)
   const unsigned *bnear;
   int *done_host;  // pinned host memory: the stop test stores the finished step's tag
+  unsigned long long *prog_host;  // pinned: (step tag << 32) | iterations tested
   // tables
   const float *wt, *gt;
   bool lds_tables;

@@ -762,6 +762,15 @@ __device__ __forceinline__ void block_residual_store(const Bufs &b, unsigned lon
 
 // Sum of the block partials of iteration `slot` into DevState; with
 // `check` set, also the stop test of the device-driven loop.
+// Progress of the device-driven loop for the host, which paces its enqueues
+// on it: the step's tag and the iterations whose stop test has run, in one
+// 64-bit store to pinned host memory.
+__device__ __forceinline__ void report_progress(const Bufs &b, const DevState *st, int iters) {
+  if (b.prog_host)
+    *(volatile unsigned long long *)b.prog_host =
+        ((unsigned long long)(unsigned)st->step_tag << 32) | (unsigned)iters;
+}
+
 __global__ void k_residual_reduce(Bufs b, int nparts, int slot, int check, int src0, int min_it,
                                   int max_it, int force_it, double eta) {
   __shared__ unsigned long long s_sum[16];
@@ -796,6 +805,7 @@ __global__ void k_residual_reduce(Bufs b, int nparts, int slot, int check, int s
       const int cont = jacobi_continue(k, S, Cn, min_it, max_it, force_it, eta, &e);
       if (cont) {
         st->iter = k + 1;
+        report_progress(b, st, k + 1);
       } else {
         st->done = 1;
         st->iterations = k + 1;
@@ -804,7 +814,7 @@ __global__ void k_residual_reduce(Bufs b, int nparts, int slot, int check, int s
         st->converged = e <= eta;
         st->final_src = src0 ^ (k & 1);
         st->done_tag = st->step_tag;
-        if (b.done_host) *(volatile int *)b.done_host = st->step_tag;
+    if (b.done_host) *(volatile int *)b.done_host = st->step_tag;
       }
     }
   }
@@ -884,7 +894,10 @@ __global__ void k_batch_stop(Bufs b, int src0, int min_it, int max_it, int force
   const int any = batch_decide(b, k, src0 ^ (k & 1), min_it, max_it, force_it, eta, s_done, true);
   for (int sc = threadIdx.x; sc < b.nscenes; sc += blockDim.x) b.sc_done[sc] = s_done[sc];
   if (any) {
-    if (threadIdx.x == 0) st->iter = k + 1;
+    if (threadIdx.x == 0) {
+      st->iter = k + 1;
+      report_progress(b, st, k + 1);
+    }
     return;
   }
   __syncthreads();
@@ -918,7 +931,7 @@ __global__ void k_batch_stop(Bufs b, int src0, int min_it, int max_it, int force
     st->converged = conv;
     st->final_src = src0 ^ (k & 1);
     st->done_tag = st->step_tag;
-        if (b.done_host) *(volatile int *)b.done_host = st->step_tag;
+    if (b.done_host) *(volatile int *)b.done_host = st->step_tag;
   }
 }
 
