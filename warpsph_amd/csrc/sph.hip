@@ -306,7 +306,7 @@ struct ListWriter {
 // the writer in ascending candidate order (the order of a one-by-one scan).
 // self: the index to skip (the particle itself), -1 for none.
 __device__ __forceinline__ void emit_batch(ListWriter &w, int j, const bool (&hit)[kBatch]) {
-  if (w.room(kBatch)) {
+  if (__all(w.room(kBatch))) {  // one wave-uniform branch in the usual case
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) w.put_fast(j + u, hit[u]);
   } else {
@@ -762,15 +762,6 @@ __device__ __forceinline__ void block_residual_store(const Bufs &b, unsigned lon
 
 // Sum of the block partials of iteration `slot` into DevState; with
 // `check` set, also the stop test of the device-driven loop.
-// Progress of the device-driven loop for the host, which paces its enqueues
-// on it: the step's tag and the iterations whose stop test has run, in one
-// 64-bit store to pinned host memory.
-__device__ __forceinline__ void report_progress(const Bufs &b, const DevState *st, int iters) {
-  if (b.prog_host)
-    *(volatile unsigned long long *)b.prog_host =
-        ((unsigned long long)(unsigned)st->step_tag << 32) | (unsigned)iters;
-}
-
 __global__ void k_residual_reduce(Bufs b, int nparts, int slot, int check, int src0, int min_it,
                                   int max_it, int force_it, double eta) {
   __shared__ unsigned long long s_sum[16];
@@ -805,7 +796,6 @@ __global__ void k_residual_reduce(Bufs b, int nparts, int slot, int check, int s
       const int cont = jacobi_continue(k, S, Cn, min_it, max_it, force_it, eta, &e);
       if (cont) {
         st->iter = k + 1;
-        report_progress(b, st, k + 1);
       } else {
         st->done = 1;
         st->iterations = k + 1;
@@ -894,10 +884,7 @@ __global__ void k_batch_stop(Bufs b, int src0, int min_it, int max_it, int force
   const int any = batch_decide(b, k, src0 ^ (k & 1), min_it, max_it, force_it, eta, s_done, true);
   for (int sc = threadIdx.x; sc < b.nscenes; sc += blockDim.x) b.sc_done[sc] = s_done[sc];
   if (any) {
-    if (threadIdx.x == 0) {
-      st->iter = k + 1;
-      report_progress(b, st, k + 1);
-    }
+    if (threadIdx.x == 0) st->iter = k + 1;
     return;
   }
   __syncthreads();
@@ -983,6 +970,17 @@ __global__ void k_j2(Bufs b, SphConst c, int src, int slot) {
 }
 
 // ---- device-resident Jacobi loop control ----------------------------------
+// Progress of the loop for the host, which paces its enqueues on it: the
+// step's tag and the iterations whose stop test has run, in one 64-bit store
+// to pinned host memory. J1 of iteration k > 0 stores it (k tests have run),
+// so the store's round trip over the host link overlaps J1's own work
+// instead of delaying the end of the one-block stop kernel.
+__device__ __forceinline__ void report_progress(const Bufs &b, const DevState *st, int iters) {
+  if (b.prog_host)
+    *(volatile unsigned long long *)b.prog_host =
+        ((unsigned long long)(unsigned)st->step_tag << 32) | (unsigned)iters;
+}
+
 // Each iteration is J1, J2 and a one-thread check; the kernels read the
 // iteration counter and the done flag from DevState and return at once when
 // the loop has finished, so the host can enqueue iterations ahead without
@@ -992,6 +990,7 @@ __global__ void k_j1_dev(Bufs b, SphConst c, int src0) {
   if (__hip_atomic_load(&b.st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const Tabs T = stage_tabs<kLds>(b, c);
   const int k = b.st->iter;
+  if (k > 0 && blockIdx.x == 0 && threadIdx.x == 0) report_progress(b, b.st, k);
   const int i = particle_index();
   const float dt = (float)b.st->dt;
   const float t3 = c.mass * (dt * dt);
