@@ -244,6 +244,55 @@ def test_drivers_agree_at_full_size():
             assert_same(f"mode {mode}", a, b)
 
 
+SHAPES = [dict(block=b, cell_layout=l, chunk_iters=ch, graph_iters=g)
+          for b, l, ch, g in ((128, config.LAYOUT_MORTON, 1, 0), (256, config.LAYOUT_LINEAR, 2, 0),
+                              (512, config.LAYOUT_MORTON, 1, 0), (512, config.LAYOUT_LINEAR, 4, 0),
+                              (512, config.LAYOUT_MORTON, 2, 2), (1024, config.LAYOUT_MORTON, 1, 0))]
+
+
+@pytest.mark.parametrize("warm", [config.WARM_PREVIOUS, config.WARM_LEARNED])
+def test_launch_shapes_agree(orc, warm):
+    """The launch shapes the autotuner chooses from (block 128 to 1024, Morton
+    or linear cells, device chunks of 1, 2 or 4 iterations, a captured graph)
+    give the same states bit for bit: the J2 LDS reservation, the list ring
+    per wave and the XCD tile mapping depend on the block size, the results
+    may not. With the previous-step start every shape also equals the oracle;
+    with the learned start (whose network runs only on the device) they equal
+    each other, start sources and iteration counts included."""
+    from warpsph_amd.mlp import PressureNet
+    from warpsph_amd.solver import IISPH
+    cfg = config.dambreak_8k().replace(jitter=0.01, warm_start=warm)
+    sc = scene.build(cfg)
+    net = None
+    if warm == config.WARM_LEARNED:
+        net = PressureNet.load(str(__import__("pathlib").Path(__file__).resolve().parents[1]
+                                   / "warpsph_amd" / "pressure_net.npy"), max_batch=1024)
+    o = orc.OracleSolver(cfg, sc) if warm == config.WARM_PREVIOUS else None
+    ref = None
+    for shape in SHAPES:
+        _announce(f"launch shape {shape}, start {warm}")
+        s = IISPH(stage_path=None, config=cfg.replace(**shape), scene=sc, collect_stats=True,
+                  predictor=net)
+        its = []
+        for _ in range(12):
+            s.step()
+            its.append(s.step_stats()["iterations"])
+        got = (its, [r["start_source"] for r in s.history()], s.x.cpu().numpy(),
+               s.v.cpu().numpy(), s.p.cpu().numpy())
+        s.close()
+        if ref is None:
+            ref = got
+            if o is not None:
+                infos = [o.step() for _ in range(12)]
+                assert its == [q["iterations"] for q in infos]
+                assert_same("x vs oracle", got[2], o.x)
+                assert_same("v vs oracle", got[3], o.v)
+                assert_same("p vs oracle", got[4], infos[-1]["p"])
+        assert got[0] == ref[0] and got[1] == ref[1], (shape, got[0], ref[0])
+        for name, a_, b_ in zip(("x", "v", "p"), got[2:], ref[2:]):
+            assert_same(f"{name} {shape}", a_, b_)
+
+
 @pytest.mark.parametrize("warm", [config.WARM_ZERO, config.WARM_PREVIOUS])
 def test_dambreak_200_step_statistics(orc, warm):
     """200 steps of the 8,000-particle dam break: per-step Jacobi iteration
