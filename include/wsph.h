@@ -219,7 +219,11 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n);
  * order). Used with WSPH_WARM_EXTERNAL. */
 int wsph_set_pressure(wsph_solver *s, const float *p);
 
-/* One full time step: IISPH.step (IISPH.py:1002-1371). */
+/* One full time step: IISPH.step (IISPH.py:1002-1371). With the
+ * device-driven Jacobi loop the call returns once the loop has ended on the
+ * device and the integration is enqueued; meanwhile the calling thread polls
+ * two words in pinned host memory (busy-waiting with a pause instruction),
+ * so that no event sits between the loop's kernels. */
 int wsph_step(wsph_solver *s, wsph_step_stats *stats);
 
 /* The four phases of IISPH.step as the reference times them

@@ -402,7 +402,7 @@ int wsph_selftest_arith(unsigned long long *out) {
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(out, d, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
-  hipFree(d);
+  (void)hipFree(d);
   HIP_TRY(e);
   return WSPH_OK;
 }

@@ -464,7 +464,6 @@ __global__ void k_density(Bufs b, SphConst c, int fuse_vadv) {
   const Tabs T = stage_tabs<kLds>(b, c);
   const int i = particle_index();
   if (i >= b.n) return;
-  const int n = b.n;
   const float4 p = b.pos[i];
   const float4 *__restrict__ pos = b.pos;
   const float4 *__restrict__ bpos = b.bpos;
@@ -508,7 +507,6 @@ __global__ void k_vadv(Bufs b, SphConst c) {
   const Tabs T = stage_tabs<kLds>(b, c);
   const int i = particle_index();
   if (i >= b.n) return;
-  const int n = b.n;
   const float dt = (float)b.st->dt;
   const float4 p = b.pos[i];
   const float4 vi = b.vel[i];
@@ -581,7 +579,6 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
   const Tabs T = stage_tabs<kLds>(b, c);
   const int i = particle_index();
   if (i >= b.n) return;
-  const int n = b.n;
   const float dt = (float)b.st->dt;
   const float dt2 = dt * dt;
   const float t3 = c.mass * dt2;
@@ -642,7 +639,6 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
 template <typename TT>
 __device__ __forceinline__ void j1_particle(const Bufs &b, const SphConst &c, const TT &T, int i,
                                             int src, float t3) {
-  const int n = b.n;
   const float4 p = b.pos[i];
   const float4 *__restrict__ dp = b.dp[src];
   const float4 *__restrict__ pos = b.pos;
@@ -676,7 +672,6 @@ __device__ __forceinline__ void j1_particle(const Bufs &b, const SphConst &c, co
 template <typename TT>
 __device__ __forceinline__ void j2_particle(const Bufs &b, const SphConst &c, const TT &T, int i,
                                             int src, unsigned long long &rsum, unsigned &rcnt) {
-  const int n = b.n;
   const float4 p = b.pos[i];
   const float4 si = b.sap[i];
   const float4 *__restrict__ pos = b.pos;
