@@ -107,6 +107,41 @@ def traffic(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_bytes):
     return out
 
 
+SQ_COLUMNS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD",
+              "SQ_WAVE_CYCLES", "SQ_WAIT_INST_ANY", "SQ_BUSY_CYCLES")
+
+
+def sq(db: str) -> str:
+    """Per kernel, from its dispatch with the most VALU instructions (a working
+    one): waves, then VALU, SALU, LDS and vector-read instructions, wave cycles
+    and wait cycles per wave, and the dispatch's busy cycles."""
+    cur = sqlite3.connect(db).cursor()
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    names = {}
+    for d, k, c, v in cur.execute(
+            "select dispatch_id, kernel_name, counter_name, value from counters_collection"):
+        per[d][c] += v
+        names[d] = k.split("(")[0]
+    best = {}
+    for d, cs in per.items():
+        k = names[d]
+        if cs.get("SQ_WAVES", 0) <= 0:
+            continue
+        if k not in best or cs.get("SQ_INSTS_VALU", 0) > best[k].get("SQ_INSTS_VALU", 0):
+            best[k] = cs
+    out = [f"{'kernel':36s} {'waves':>8s} {'valu/w':>8s} {'salu/w':>8s} {'lds/w':>7s} "
+           f"{'vmem/w':>7s} {'wcyc/w':>8s} {'wait/w':>8s} {'busy':>9s}"]
+    for k, cs in sorted(best.items(), key=lambda kv: -kv[1].get("SQ_BUSY_CYCLES", 0)):
+        w = cs["SQ_WAVES"]
+        if w < 1000:
+            continue
+        out.append(f"{k[:36]:36s} {w:8.0f} {cs['SQ_INSTS_VALU'] / w:8.0f} "
+                   f"{cs['SQ_INSTS_SALU'] / w:8.0f} {cs['SQ_INSTS_LDS'] / w:7.0f} "
+                   f"{cs['SQ_INSTS_VMEM_RD'] / w:7.0f} {cs['SQ_WAVE_CYCLES'] / w:8.0f} "
+                   f"{cs['SQ_WAIT_INST_ANY'] / w:8.0f} {cs['SQ_BUSY_CYCLES']:9.0f}")
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -116,6 +151,8 @@ def main():
     dsp = sub.add_parser("dispatches")
     dsp.add_argument("csv")
     dsp.add_argument("--steps", type=int)
+    q = sub.add_parser("sq")
+    q.add_argument("db")
     t = sub.add_parser("traffic")
     t.add_argument("fetch")
     t.add_argument("write")
@@ -127,6 +164,8 @@ def main():
         print(kernels(a.db, a.steps))
     elif a.cmd == "dispatches":
         print(dispatches(a.csv, a.steps))
+    elif a.cmd == "sq":
+        print(sq(a.db))
     else:
         print(json.dumps(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes),
                          indent=1))
