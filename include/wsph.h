@@ -258,10 +258,12 @@ int wsph_hash_keys(const float *x, int n, float cell_width, const int dims[3],
  * whole steps, out[5] persistent-kernel grid size, out[6] steps recorded,
  * out[7] and out[8] ms of the J1 and J2 launches (A p, IISPH.py:431-518) of
  * each step's first Jacobi iteration (device-driven loop without a captured
- * graph), out[9] steps in those two sums. wsph_set_timing(s, 2) records only
- * the J1 and J2 events, on every 4th step (out[0..4] stay 0): each event
- * between two kernels costs a few microseconds of device time, which mode 1
- * adds to every step. 0 switches timing off. */
+ * graph), out[9] steps in those two sums, out[10] ms of the same J1 + J2
+ * pairs from the event before J1 to the event after J2, out[11] pairs timed.
+ * wsph_set_timing(s, 2) records only the two events around the pair, on
+ * every 4th step (out[0..4] and out[7..9] stay 0): each event between two
+ * kernels costs a few microseconds of device time, which mode 1 adds to every
+ * step and to each of the J1 and J2 times. 0 switches timing off. */
 int wsph_last_timings(wsph_solver *s, double *out, int nout);
 int wsph_set_timing(wsph_solver *s, int enable);
 

@@ -61,8 +61,9 @@ def test_graph_step_zero_start_and_timing():
 
 
 def test_kernel_timing_mode():
-    """Timing mode 2 records only the first iteration's J1/J2 events, on
-    every 4th step, and leaves the phase sums at zero; the device-driven loop
+    """Timing mode 2 records only the two events around the first
+    iteration's J1 + J2 pair, on every 4th step, and leaves the phase and the
+    per-kernel sums at zero; the device-driven loop
     paces itself on the pinned progress word, with the same results as an
     untimed solver."""
     from warpsph_amd import _native as N
@@ -78,10 +79,10 @@ def test_kernel_timing_mode():
         b.step()
     assert_same("x", b.x.cpu().numpy(), a.x.cpu().numpy())
     assert b.counters() == a.counters()
-    tm = (C.c_double * 10)()
-    N.check("wsph_last_timings", b._lib.wsph_last_timings(b._h, tm, 10))
+    tm = (C.c_double * 12)()
+    N.check("wsph_last_timings", b._lib.wsph_last_timings(b._h, tm, 12))
     assert tm[6] == 12 and tm[4] == 0 and tm[0] == 0
-    assert tm[9] == 3 and tm[7] > 0 and tm[8] > 0
+    assert tm[9] == 0 and tm[11] == 3 and tm[10] > 0
 
 
 def test_graph_step_learned_start():

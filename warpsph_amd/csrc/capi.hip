@@ -214,6 +214,8 @@ struct wsph_solver {
   bool kev_rec[kRing] = {};
   double k_sum[2] = {0, 0};
   long k_steps = 0;
+  double k_pair = 0;  // J1 + J2 between two events (the first before J1, the second after J2)
+  long k_pairs = 0;
   long t_recorded = 0, t_processed = 0;
   double t_sum[kEv] = {0};
   int persistent_grid = 0;
@@ -820,11 +822,16 @@ static void timing_fold_one(wsph_solver *s) {
   }
   if (s->kev_rec[slot]) {
     float m1, m2;
-    if (hipEventElapsedTime(&m1, s->ev[base + 5], s->ev[base + 6]) == hipSuccess &&
+    if (s->timing == 1 &&
+        hipEventElapsedTime(&m1, s->ev[base + 5], s->ev[base + 6]) == hipSuccess &&
         hipEventElapsedTime(&m2, s->ev[base + 6], s->ev[base + 7]) == hipSuccess) {
       s->k_sum[0] += m1;
       s->k_sum[1] += m2;
       s->k_steps++;
+    }
+    if (hipEventElapsedTime(&m1, s->ev[base + 5], s->ev[base + 7]) == hipSuccess) {
+      s->k_pair += m1;
+      s->k_pairs++;
     }
     s->kev_rec[slot] = false;
   }
@@ -841,6 +848,7 @@ static void kernel_events(wsph_solver *s, hipEvent_t ev3[3]) {
   const int slot = (int)(s->t_recorded % wsph_solver::kRing);
   const int base = slot * wsph_solver::kEv;
   for (int k = 0; k < 3; ++k) ev3[k] = s->ev[base + 5 + k];
+  if (s->timing == 2) ev3[1] = nullptr;  // the pair only: one event less inside it
   s->kev_rec[slot] = true;
 }
 
@@ -1681,6 +1689,8 @@ int wsph_set_timing(wsph_solver *s, int enable) {
   for (double &v : s->t_sum) v = 0.0;
   s->k_sum[0] = s->k_sum[1] = 0.0;
   s->k_steps = 0;
+  s->k_pair = 0.0;
+  s->k_pairs = 0;
   for (bool &f : s->kev_rec) f = false;
   return WSPH_OK;
 }
@@ -1688,10 +1698,10 @@ int wsph_set_timing(wsph_solver *s, int enable) {
 int wsph_last_timings(wsph_solver *s, double *out, int nout) {
   if (!s || !out) return fail(WSPH_E_ARG, "null argument");
   while (s->t_processed < s->t_recorded) timing_fold_one(s);
-  double v[10] = {s->t_sum[0], s->t_sum[1], s->t_sum[2], s->t_sum[3], s->t_sum[4],
+  double v[12] = {s->t_sum[0], s->t_sum[1], s->t_sum[2], s->t_sum[3], s->t_sum[4],
                   (double)s->persistent_grid, (double)s->t_processed, s->k_sum[0], s->k_sum[1],
-                  (double)s->k_steps};
-  for (int k = 0; k < nout && k < 10; ++k) out[k] = v[k];
+                  (double)s->k_steps, s->k_pair, (double)s->k_pairs};
+  for (int k = 0; k < nout && k < 12; ++k) out[k] = v[k];
   return WSPH_OK;
 }
 

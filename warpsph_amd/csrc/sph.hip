@@ -1514,7 +1514,7 @@ void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P,
     const bool ev = ev3 && ev3[0] && k == 0;
     if (ev) (void)hipEventRecord(ev3[0], s);
     DISPATCH_LDS(k_j1_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0);
-    if (ev) (void)hipEventRecord(ev3[1], s);
+    if (ev && ev3[1]) (void)hipEventRecord(ev3[1], s);
     DISPATCH_LDS(k_j2_dev, nblocks(b.n, block), block, j2_lds_bytes(b, c, block), s, b, c, src0);
     if (ev) (void)hipEventRecord(ev3[2], s);
     if (b.scenes)
