@@ -140,11 +140,14 @@ def cpu_baseline(cfg, x, v, budget_s: float, threads: int = 1, boundary=None):
         if el >= budget_s * 0.5 or el * (steps + 1) / steps > budget_s:
             break
     el = time.perf_counter() - t0
+    dims = scn.grid_dims_for(cfg, sc)[0]
+    grid = (f"the reference's hash dims {dims[0]}x{dims[1]}x{dims[2]} (int(BOX/h/4))"
+            if cfg.grid == "reference" else "domain-covering hash dims")
     return {"value": cfg_n(x) * steps / el, "unit": "fluid particle-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"{cfg_n(x)}-particle state at the end of the GPU run, {steps} step(s), "
                       f"{el:.1f} s, {threads} thread(s) (OpenMP), float32 oracle restating the "
-                      f"reference step with domain-covering hash dims"}
+                      f"reference step with {grid}"}
 
 
 def cfg_n(x):
@@ -546,6 +549,13 @@ def main():
         par["host_cpu"] = _cpu_model()
         par["nproc"] = _nproc()
         line["cpu_baseline_all_cores"] = par
+        # the reference's own hash dims (IISPH.py:771-780): their aliased cells hold many
+        # more candidates per query; one bounded sample on every core
+        ref = cpu_baseline(ccfg.replace(grid="reference"), x_end, v_end, args.cpu_seconds / 2,
+                           threads=_nproc(), boundary=b_end)
+        ref["host_cpu"] = _cpu_model()
+        ref["nproc"] = _nproc()
+        line["cpu_reference_dims_all_cores"] = ref
     print(json.dumps(line), flush=True)
 
 
