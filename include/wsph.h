@@ -170,6 +170,21 @@ int wsph_abi_version(void);
  * d2 < wsph_radius_sq_threshold(h). */
 float wsph_radius_sq_threshold(float h);
 
+/* Blocks of block_threads threads that fit on one compute unit at the same
+ * time (no counterpart in the reference; it sizes the grid of the persistent
+ * Jacobi kernel, whose grid-wide barrier needs every block resident): the
+ * minimum over the wave slots (waves_per_simd on each of the 4 SIMDs, each
+ * block counted as taking ceil(waves / 4) slots on every SIMD: exact for
+ * blocks of 4 or more waves, a lower bound for smaller ones), the VGPRs
+ * (512 per lane and SIMD, allocated in units of 8, vgprs per lane of the
+ * kernel), the LDS (lds_bytes per block, static plus dynamic, allocated in
+ * units of 512 bytes, against cu_lds_bytes) and 16 workgroups per CU. Host
+ * arithmetic only; WSPH_E_ARG for figures out of range. The persistent
+ * driver reads the kernel's figures with hipFuncGetAttributes and launches
+ * one block per CU fewer than this (at least one). */
+int wsph_blocks_per_cu(int block_threads, int vgprs, int lds_bytes, int cu_lds_bytes,
+                       int waves_per_simd);
+
 /* Self-test of the kernels' rounding helpers on the current device (no
  * counterpart in the reference, which leaves float32 division and sqrt to
  * Warp's codegen). The neighbour loops replace the compiler's correctly

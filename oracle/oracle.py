@@ -6,6 +6,7 @@ and bench.py's cpu_baseline leg import this module.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import subprocess
 from pathlib import Path
@@ -49,25 +50,43 @@ class OrcOut(C.Structure):
     ]
 
 
+def _load(path: Path) -> C.CDLL:
+    from warpsph_amd._native import WsphParams
+    L = C.CDLL(str(path))
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_hash_keys.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p]
+    L.orc_neighbors.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_float,
+                                C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                C.c_void_p]
+    L.orc_boundary_psi.argtypes = [C.POINTER(WsphParams), C.c_void_p, C.c_int, C.c_int,
+                                   C.c_void_p, C.c_void_p]
+    L.orc_step.argtypes = [C.POINTER(WsphParams), C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                           C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                           C.POINTER(OrcOut)]
+    return L
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
         build()
-        from warpsph_amd._native import WsphParams
-        L = C.CDLL(str(LIB))
-        L.orc_set_threads.argtypes = [C.c_int]
-        L.orc_hash_keys.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p]
-        L.orc_neighbors.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_float,
-                                    C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
-                                    C.c_void_p]
-        L.orc_boundary_psi.argtypes = [C.POINTER(WsphParams), C.c_void_p, C.c_int, C.c_int,
-                                       C.c_void_p, C.c_void_p]
-        L.orc_step.argtypes = [C.POINTER(WsphParams), C.c_void_p, C.c_int, C.c_void_p, C.c_int,
-                               C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
-                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
-                               C.POINTER(OrcOut)]
-        _lib = L
+        _lib = _load(LIB)
     return _lib
+
+
+@contextlib.contextmanager
+def using(path):
+    """Route every call of this module through another build of the oracle
+    for the duration of the block (the tests build copies with planted
+    mistakes to show that the pins of each function catch them)."""
+    global _lib
+    old = _lib
+    _lib = _load(Path(path))
+    try:
+        yield
+    finally:
+        _lib = old
 
 
 def set_threads(n: int) -> None:

@@ -2,6 +2,8 @@
 on the same scene and compare their fields."""
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from warpsph_amd import config, scene, tables
@@ -51,3 +53,43 @@ def assert_same(name, a, b):
                 f"{a[tuple(k)]!r} vs {b[tuple(k)]!r}; max abs diff {np.nanmax(diff):.3e}")
     else:
         assert np.array_equal(a, b), f"{name}: integer arrays differ"
+
+
+# ---- the cubic spline in closed form (float64), independent of the tables --
+def kernel_sigma(h):
+    """sigma = 8 / (pi h^3) (wp_sph/main.py:60)."""
+    return 8.0 / (math.pi * h ** 3)
+
+
+def kernel_w(r, h):
+    """W(r) = sigma [6 (q^3 - q^2) + 1] for q <= 1/2, 2 sigma (1 - q)^3 for
+    q < 1, else 0, q = r / h."""
+    q = r / h
+    s = kernel_sigma(h)
+    if q <= 0.5:
+        return s * (6 * (q ** 3 - q ** 2) + 1)
+    if q < 1:
+        return 2 * s * (1 - q) ** 3
+    return 0.0
+
+
+def kernel_dw(r, h):
+    """|dW/dr| of the cubic spline."""
+    q = r / h
+    s = kernel_sigma(h) / h
+    if q <= 0.5:
+        return abs(s * (18 * q * q - 12 * q))
+    if q < 1:
+        return abs(-6 * s * (1 - q) ** 2)
+    return 0.0
+
+
+def kernel_grad(xi, xj, h):
+    """grad_spline_W(x_j - x_i) (sph_funcs.py:105-118) in closed form: the
+    table holds -W'(r) / r, so the value is |W'(r)| times the unit vector
+    from x_i to x_j."""
+    r = np.asarray(xj, np.float64) - np.asarray(xi, np.float64)
+    n = float(np.linalg.norm(r))
+    if not 1e-10 < n < h:
+        return np.zeros(3)
+    return kernel_dw(n, h) * r / n

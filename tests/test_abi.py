@@ -61,3 +61,31 @@ def test_radius_threshold_matches_sqrt_predicate(h):
     rnd = np.random.default_rng(1).uniform(0, 4 * float(t), 200000).astype(np.float32)
     for d2 in (near, rnd):
         assert np.array_equal(np.sqrt(d2) < hf, d2 < t)
+
+
+def test_blocks_per_cu_from_kernel_resources():
+    """The co-residency figure of wsph_blocks_per_cu (host arithmetic, no GPU),
+    on the persistent Jacobi kernel's own resource figures (-Rpass-analysis
+    =kernel-resource-usage of the gfx950 build) and on limits taken one at a
+    time (MI355X: 160 KiB LDS per CU, 8 waves per SIMD)."""
+    lib = N.load()
+    lds_cu, wps = 160 * 1024, 8
+    # an earlier build of the kernel: 62 VGPRs, 200 B static + 8,000 B of
+    # tables, 256-thread blocks: the wave slots bound it at 8 per CU, where a
+    # launch once found only 7 resident, so the driver launches at most 7
+    assert lib.wsph_blocks_per_cu(256, 62, 200 + 8000, lds_cu, wps) == 8
+    # the current kernel: 114 VGPRs (4 waves per SIMD), 8,648 B static + 8,000 B
+    # of tables; 512-thread blocks put 2 waves on each SIMD
+    assert lib.wsph_blocks_per_cu(512, 114, 8648 + 8000, lds_cu, wps) == 2
+    assert lib.wsph_blocks_per_cu(256, 114, 8648 + 8000, lds_cu, wps) == 4
+    # LDS: 60,000 B rounds up to 60,416 B, two fit in 160 KiB
+    assert lib.wsph_blocks_per_cu(64, 32, 60000, lds_cu, wps) == 2
+    # VGPRs allocated in units of 8: 129 -> 136 -> 3 waves per SIMD
+    assert lib.wsph_blocks_per_cu(256, 129, 0, lds_cu, wps) == 3
+    # 16 workgroups per CU at most; one-wave blocks counted on every SIMD (a
+    # lower bound: they could pack 4 times as densely)
+    assert lib.wsph_blocks_per_cu(64, 16, 0, lds_cu, wps) == 8
+    assert lib.wsph_blocks_per_cu(64, 16, 0, lds_cu, 32) == 16
+    # a block of 1024 threads (16 waves) needs 4 waves on each SIMD
+    assert lib.wsph_blocks_per_cu(1024, 64, 0, lds_cu, wps) == 2
+    assert lib.wsph_blocks_per_cu(0, 64, 0, lds_cu, wps) == N.WSPH_E_ARG

@@ -80,3 +80,33 @@ def test_batch_of_one_scene_matches_solo():
     assert_same("x", b.x.cpu().numpy(), s.x.cpu().numpy())
     b.close()
     s.close()
+
+
+@pytest.mark.parametrize("jacobi,start", [
+    (config.JACOBI_DEVICE, config.WARM_PREVIOUS),
+    (config.JACOBI_PERSISTENT, config.WARM_PREVIOUS),
+    (config.JACOBI_DEVICE, config.WARM_ZERO),
+])
+def test_batch_scenes_follow_oracle(orc, jacobi, start):
+    """Each scene of a 6-scene batch drawn from the C3 scene generator
+    (batch_configs) follows the CPU oracle's IISPH.step (IISPH.py:1002-1371)
+    bit for bit: x, v, p after every step and the scene's Jacobi iterations
+    per step."""
+    from warpsph_amd.batch import BatchIISPH
+    steps = 10
+    cfgs = [c.replace(jacobi_mode=jacobi, warm_start=start) for c in _scene_cfgs(6, 12000)]
+    _announce(f"batch of {len(cfgs)} scenes against the oracle, jacobi={jacobi} start={start}")
+    scs = [scene.build(c) for c in cfgs]
+    b = BatchIISPH(cfgs, scs)
+    orcs = [orc.OracleSolver(c, sc) for c, sc in zip(cfgs, scs)]
+    for k in range(steps):
+        b.step()
+        iters = b.scene_stats()["iterations"]
+        xs, vs, ps = (b.split(t.cpu().numpy()) for t in (b.x, b.v, b.p))
+        for q, o in enumerate(orcs):
+            info = o.step()
+            assert iters[q] == info["iterations"], (k, q, iters[q], info["iterations"])
+            assert_same(f"step {k} scene {q} x", xs[q], o.x)
+            assert_same(f"step {k} scene {q} v", vs[q], o.v)
+            assert_same(f"step {k} scene {q} p", ps[q], o.p)
+    b.close()

@@ -9,32 +9,10 @@ import numpy as np
 import pytest
 
 from helpers import brute_force_lists, oracle_setup
+from helpers import kernel_dw as dw_exact
+from helpers import kernel_sigma as sigma
+from helpers import kernel_w as w_exact
 from warpsph_amd import config, scene, tables
-
-
-def sigma(h):
-    return 8.0 / (math.pi * h ** 3)
-
-
-def w_exact(r, h):
-    q = r / h
-    s = sigma(h)
-    if q <= 0.5:
-        return s * (6 * (q ** 3 - q ** 2) + 1)
-    if q < 1:
-        return 2 * s * (1 - q) ** 3
-    return 0.0
-
-
-def dw_exact(r, h):
-    """|dW/dr| of the cubic spline."""
-    q = r / h
-    s = sigma(h) / h
-    if q <= 0.5:
-        return abs(s * (18 * q * q - 12 * q))
-    if q < 1:
-        return abs(-6 * s * (1 - q) ** 2)
-    return 0.0
 
 
 # ---- tables (sph_funcs.py:51-118) ---------------------------------------
@@ -165,8 +143,10 @@ def test_interior_density_closed_form(orc):
     assert abs(rho_exact / cfg.rho0 - 0.99997247) < 1e-8
     k = (6 * 12 + 6) * 12 + 6
     assert abs(info["rho"][k] / rho_exact - 1) < 1e-5
-    # Sum_j grad W = 0 in the interior: d_ii = 0 and rho_adv = rho at rest
+    # Sum_j grad W = 0 in the interior: d_ii = 0; and at rest v_adv = g dt is
+    # the same for every particle, so rho_adv = rho exactly away from the walls
     assert np.all(np.abs(info["d_ii"][k]) < 1e-12)
+    assert info["rho_adv"][k] == info["rho"][k]
     # a_ii = -(rho0/rho)^2 (dt/d)^2 * sum |W'|^2 h^2 / (4 pi^2 sigma^2 h^2 ...)
     s = (6 * 1.5 ** 2 + 12 * (6 * (1 - 1 / math.sqrt(2)) ** 2) ** 2
          + 8 * (6 * (1 - math.sqrt(3) / 2) ** 2) ** 2) / (4 * math.pi ** 2)

@@ -135,7 +135,7 @@ void launch_batch_finalize(const Bufs &b, hipStream_t s);
 // Persistent cooperative Jacobi loop; returns a hip error code.
 hipError_t launch_jacobi_persistent(const Bufs &b, const SphConst &c, const wsph_params &P,
                                     int src, int grid, int block, hipStream_t s);
-int persistent_max_blocks(const Bufs &b, const SphConst &c, int block, int *blocks_per_cu);
+int persistent_blocks_per_cu(const Bufs &b, const SphConst &c, int block, int *total);
 // Field scatter back to original order
 void launch_scatter(const void *src, int stride_floats, int ncomp, int src_off, const int *orig,
                     int n, void *dst, hipStream_t s);

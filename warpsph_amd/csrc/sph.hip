@@ -1548,20 +1548,28 @@ void launch_reset_state(const Bufs &b, hipStream_t s) {
   if (b.scenes) k_batch_reset<<<64, 256, 0, s>>>(b, kMaxIters);
 }
 
-int persistent_max_blocks(const Bufs &b, const SphConst &c, int block, int *blocks_per_cu) {
-  int dev = 0, ncu = 0, per = 0;
+// Blocks of k_jacobi_persistent that fit on one CU at once, from the kernel's
+// attributes (VGPRs, static LDS -- the barrier's and the residual's
+// __shared__ words -- plus the dynamic table LDS) and the device's limits;
+// *total = that figure times the number of CUs.
+int persistent_blocks_per_cu(const Bufs &b, const SphConst &c, int block, int *total) {
+  int dev = 0, ncu = 0, lds_cu = 0, threads_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return -1;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&threads_cu, hipDeviceAttributeMaxThreadsPerMultiProcessor, dev) !=
+          hipSuccess)
     return -1;
-  const size_t shm = tab_bytes(b, c);
-  hipError_t e;
-  if (b.lds_tables)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_jacobi_persistent<true>, block, shm);
-  else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_jacobi_persistent<false>, block, shm);
+  hipFuncAttributes fa;
+  const hipError_t e = b.lds_tables
+                           ? hipFuncGetAttributes(&fa, (const void *)k_jacobi_persistent<true>)
+                           : hipFuncGetAttributes(&fa, (const void *)k_jacobi_persistent<false>);
   if (e != hipSuccess) return -1;
-  if (blocks_per_cu) *blocks_per_cu = per;
-  return per * ncu;
+  const int lds = (int)(fa.sharedSizeBytes + tab_bytes(b, c));
+  const int per = wsph_blocks_per_cu(block, fa.numRegs, lds, lds_cu, threads_cu / 64 / 4);
+  if (total) *total = per * ncu;
+  return per;
 }
 
 hipError_t launch_jacobi_persistent(const Bufs &b, const SphConst &c, const wsph_params &P,
