@@ -322,6 +322,22 @@ int wsph_poisson_surface(int shape, const double *params, double radius, uint64_
  * double[nx]/[ny]/[nz], inside uint8[nx * ny * nz]. Blocks until done. */
 int wsph_mesh_inside(const double *verts, int nv, const int *tris, int nt, const double *ax, int nx,
                      const double *ay, int ny, const double *az, int nz, unsigned char *inside);
+/* wsph_mesh_inside for callers that cannot allocate (code being captured
+ * into a HIP graph, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference. Same arguments and the same result, bit
+ * for bit; it calls no hipMalloc/hipFree, creates no stream or event and
+ * allocates no host memory. Every scratch buffer comes from one fixed
+ * 64 MiB device array per GPU, held until the process ends: 128 bytes per
+ * triangle, 16 bytes per 16 x 16 column tile, 12 bytes per lattice column
+ * (nx * ny), 4 bytes per triangle-tile pair, 16 bytes per work item of up to
+ * 32 triangles and 8 bytes per column crossing, each region rounded up to
+ * 256 bytes. A call that needs more returns WSPH_E_CAPACITY, before any
+ * device work when the sizes alone exceed it. Runs on the null stream and
+ * blocks until done; records no timings. Calls that overlap on one GPU (two
+ * host threads) race on its array: serialise them. */
+int wsph_mesh_inside_noalloc(const double *verts, int nv, const int *tris, int nt,
+                             const double *ax, int nx, const double *ay, int ny,
+                             const double *az, int nz, unsigned char *inside);
 /* Kernel durations (ms, HIP events) of the calling thread's last
  * wsph_mesh_inside, out[0..4]: triangle set-up, binning of the triangles into
  * 16 x 16 column tiles, crossing count pass, crossing fill pass, sort and

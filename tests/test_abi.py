@@ -89,3 +89,19 @@ def test_blocks_per_cu_from_kernel_resources():
     # a block of 1024 threads (16 waves) needs 4 waves on each SIMD
     assert lib.wsph_blocks_per_cu(1024, 64, 0, lds_cu, wps) == 2
     assert lib.wsph_blocks_per_cu(0, 64, 0, lds_cu, wps) == N.WSPH_E_ARG
+
+
+def test_mesh_noalloc_rejects_oversized_lattice_before_device_work():
+    """wsph_mesh_inside_noalloc sizes its fixed scratch regions from the
+    arguments alone: 9 M lattice columns need 108 MB of column counters,
+    beyond its 64 MiB array, so it returns WSPH_E_CAPACITY without touching
+    the device (the dummy pointers here are never dereferenced)."""
+    lib = N.load()
+    dummy = ctypes.c_double(0.0)
+    out = ctypes.c_ubyte(0)
+    tri = ctypes.c_int(0)
+    p = ctypes.addressof(dummy)
+    rc = lib.wsph_mesh_inside_noalloc(p, 1, ctypes.addressof(tri), 0, p, 3000, p, 3000, p, 1,
+                                      ctypes.addressof(out))
+    assert rc == N.WSPH_E_CAPACITY
+    assert b"64 MiB" in lib.wsph_last_error()

@@ -116,6 +116,7 @@ SIGNATURES = [
     ("wsph_set_fluid", _I, [_P, _P, _P, _I]),
     ("wsph_set_rigid", _I, [_P, _I, _P]),
     ("wsph_mesh_inside", _I, [_P, _I, _P, _I, _P, _I, _P, _I, _P, _I, _P]),
+    ("wsph_mesh_inside_noalloc", _I, [_P, _I, _P, _I, _P, _I, _P, _I, _P, _I, _P]),
     ("wsph_mesh_timings", _I, [_P]),
     ("wsph_set_pressure", _I, [_P, _P]),
     ("wsph_step", _I, [_P, C.POINTER(WsphStepStats)]),

@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "kernels.h"
@@ -387,6 +388,52 @@ static int check_dims(const int *d, const char *what) {
     if (d[k] < 1 || d[k] > 1024) return fail(WSPH_E_ARG, "%s dims must be in [1, 1024]", what);
   return WSPH_OK;
 }
+
+// ---- wsph_mesh_inside_noalloc: the same classification with no allocation --
+// Every scratch buffer of the classification lives in one file-scope device
+// array (each GPU the library runs on holds its own copy, at its own address,
+// until the process ends). Host code never uses the array's name as a device
+// address: hipGetSymbolAddress gives the address on each device once, and
+// the per-device pointer is kept.
+constexpr size_t kMeshScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kMeshScratchBytes];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_dev[kScratchDevices] = {};
+
+static int scratch_base(unsigned char **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) return fail(WSPH_E_UNSUPPORTED, "device %d", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(WSPH_E_HIP, "no device address for the mesh scratch array");
+    g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return WSPH_OK;
+}
+
+// Consecutive 256-byte aligned regions of the scratch array; with base null
+// it only adds up the sizes.
+struct ScratchCarve {
+  unsigned char *base;
+  size_t used = 0;
+  bool over = false;
+  template <typename T>
+  T *take(size_t count) {
+    const size_t off = (used + 255) & ~size_t(255);
+    const size_t bytes = sizeof(T) * (count > 0 ? count : 1);
+    if (off > kMeshScratchBytes || bytes > kMeshScratchBytes - off) {
+      over = true;
+      return nullptr;
+    }
+    used = off + bytes;
+    return base ? reinterpret_cast<T *>(base + off) : nullptr;
+  }
+};
 
 extern "C" {
 
@@ -1068,6 +1115,91 @@ int wsph_mesh_inside(const double *verts, int nv, const int *tris, int nt, const
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, m.ev[2 * k], m.ev[2 * k + 1]) == hipSuccess) g_mesh_ms[k] = ms;
   }
+  return WSPH_OK;
+}
+
+int wsph_mesh_inside_noalloc(const double *verts, int nv, const int *tris, int nt, const double *ax,
+                             int nx, const double *ay, int ny, const double *az, int nz,
+                             unsigned char *inside) {
+  if (nv < 0 || nt < 0 || nx < 0 || ny < 0 || nz < 0) return fail(WSPH_E_ARG, "negative size");
+  const long ncol = (long)nx * ny, npts = ncol * nz;
+  if (npts == 0) return WSPH_OK;
+  if (!inside || !ax || !ay || !az) return fail(WSPH_E_ARG, "null lattice or output pointer");
+  if (nt > 0 && (!verts || !tris || nv < 1)) return fail(WSPH_E_ARG, "null mesh arrays");
+  const long ntiles = mesh_tiles(nx, ny);
+  // the part whose size the arguments fix, checked before any device work
+  auto fixed = [&](ScratchCarve &c, int **bad, unsigned long long **tot, MeshTri **tri,
+                   unsigned **tcnt, unsigned **toff, unsigned **tpos, unsigned **ioff,
+                   unsigned **cnt, unsigned **off, unsigned **fill) {
+    *bad = c.take<int>(1);
+    *tot = c.take<unsigned long long>(2);
+    *tri = c.take<MeshTri>((size_t)nt);
+    *tcnt = c.take<unsigned>((size_t)ntiles);
+    *toff = c.take<unsigned>((size_t)ntiles);
+    *tpos = c.take<unsigned>((size_t)ntiles);
+    *ioff = c.take<unsigned>((size_t)ntiles);
+    *cnt = c.take<unsigned>((size_t)ncol);
+    *off = c.take<unsigned>((size_t)ncol);
+    *fill = c.take<unsigned>((size_t)ncol);
+  };
+  int *bad;
+  unsigned long long *tot;
+  MeshTri *tri;
+  unsigned *tcnt, *toff, *tpos, *ioff, *cnt, *off, *fill;
+  {
+    ScratchCarve sizing{nullptr};
+    fixed(sizing, &bad, &tot, &tri, &tcnt, &toff, &tpos, &ioff, &cnt, &off, &fill);
+    if (sizing.over)
+      return fail(WSPH_E_CAPACITY,
+                  "%d triangles on a %d x %d lattice need more than the %zu MiB of mesh scratch",
+                  nt, nx, ny, kMeshScratchBytes >> 20);
+  }
+  unsigned char *base = nullptr;
+  int rc = scratch_base(&base);
+  if (rc) return rc;
+  ScratchCarve c{base};
+  fixed(c, &bad, &tot, &tri, &tcnt, &toff, &tpos, &ioff, &cnt, &off, &fill);
+  hipStream_t st = nullptr;  // the null stream: nothing to create
+  fill_u32(reinterpret_cast<unsigned *>(bad), 0u, 1, st);
+  fill_u32(tcnt, 0u, ntiles, st);
+  fill_u32(tpos, 0u, ntiles, st);
+  fill_u32(cnt, 0u, ncol, st);
+  fill_u32(fill, 0u, ncol, st);
+  launch_mesh_check_axes(ax, nx, bad, st);
+  launch_mesh_check_axes(ay, ny, bad, st);
+  launch_mesh_check_axes(az, nz, bad, st);
+  launch_mesh_setup(verts, nv, tris, nt, ax, nx, ay, ny, tri, tcnt, bad, st);
+  launch_scan_excl(tcnt, ntiles, 0u, toff, tot, st);
+  launch_scan_excl(tcnt, ntiles, (unsigned)mesh_chunk(), ioff, tot + 1, st);
+  HIP_TRY(hipGetLastError());
+  unsigned long long h_tot[2] = {0, 0};
+  int h_bad = 0;
+  HIP_TRY(hipMemcpy(h_tot, tot, sizeof(h_tot), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost));
+  if (h_bad & 1) return fail(WSPH_E_ARG, "triangle vertex index outside [0, %d)", nv);
+  if (h_bad & 2) return fail(WSPH_E_ARG, "lattice axes must be strictly ascending");
+  if (h_tot[0] > 0x7fffffffull) return fail(WSPH_E_CAPACITY, "%llu triangle-tile pairs", h_tot[0]);
+  int *ids = c.take<int>((size_t)h_tot[0]);
+  int4 *items = c.take<int4>((size_t)h_tot[1]);
+  if (c.over)
+    return fail(WSPH_E_CAPACITY, "%llu triangle-tile pairs need more than the %zu MiB of mesh scratch",
+                h_tot[0], kMeshScratchBytes >> 20);
+  const int nitems = (int)h_tot[1];
+  launch_mesh_items(tcnt, toff, ioff, ntiles, items, st);
+  launch_mesh_bin(tri, nt, ny, toff, tpos, ids, st);
+  launch_mesh_columns(tri, ids, items, nitems, ax, nx, ay, ny, cnt, nullptr, nullptr, nullptr, st);
+  launch_scan_excl(cnt, ncol, 0u, off, tot, st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(h_tot, tot, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  if (h_tot[0] > 0xffffffffull) return fail(WSPH_E_CAPACITY, "%llu column crossings", h_tot[0]);
+  double *z = c.take<double>((size_t)h_tot[0]);
+  if (c.over)
+    return fail(WSPH_E_CAPACITY, "%llu column crossings need more than the %zu MiB of mesh scratch",
+                h_tot[0], kMeshScratchBytes >> 20);
+  launch_mesh_columns(tri, ids, items, nitems, ax, nx, ay, ny, cnt, off, fill, z, st);
+  launch_mesh_classify(off, cnt, z, ncol, az, nz, inside, st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
   return WSPH_OK;
 }
 

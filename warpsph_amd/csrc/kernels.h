@@ -162,6 +162,14 @@ void launch_mesh_bin(const MeshTri *tris, int nt, int ny, const unsigned *tile_o
                      unsigned *tile_pos, int *ids, hipStream_t st);
 // triangles per work item of the column passes
 int mesh_chunk();
+// out[k] = sum of the counts before k (per > 0: of ceil(count / per)) for n
+// counts, in one block; *total = the 64-bit sum
+void launch_scan_excl(const unsigned *in, long n, unsigned per, unsigned *out,
+                      unsigned long long *total, hipStream_t st);
+// work items {tile, first id, count, 0} of the column passes from the per-tile
+// triangle counts, their offsets and the tiles' first item
+void launch_mesh_items(const unsigned *tile_cnt, const unsigned *tile_off,
+                       const unsigned *item_off, long ntiles, int4 *items, hipStream_t st);
 // per column (x_i, y_j), c = i * ny + j: crossing counts added to cnt (zbuf
 // null) or heights written at off[c] + fill cursor; one block per work item
 // {tile, first id, number of ids (<= mesh_chunk())}

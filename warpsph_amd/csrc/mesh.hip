@@ -260,6 +260,71 @@ __global__ void k_mesh_mark(const unsigned *__restrict__ off, const unsigned *__
   inside[p] = (unsigned char)((m - lo) & 1);
 }
 
+// Exclusive prefix sum of n counts in one block: each thread sums one
+// contiguous chunk, the block scans the chunk sums (Hillis-Steele, 64-bit),
+// and each thread writes its chunk's offsets. With per > 0 entry k counts
+// ceil(in[k] / per) (the work items of a tile) instead of in[k]. *total
+// receives the 64-bit sum; offsets are stored in 32 bits, so a caller uses
+// them only once it has checked *total.
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void k_scan_excl(const unsigned *__restrict__ in, long n,
+                                                          unsigned per, unsigned *__restrict__ out,
+                                                          unsigned long long *__restrict__ total) {
+  __shared__ unsigned long long s[kScanBlock];
+  const long chunk = (n + kScanBlock - 1) / kScanBlock;
+  const long b = (long)threadIdx.x * chunk;
+  const long e = b + chunk < n ? b + chunk : n;
+  auto val = [&](long k) -> unsigned long long {
+    return per ? ((unsigned long long)in[k] + per - 1) / per : in[k];
+  };
+  unsigned long long sum = 0;
+  for (long k = b; k < e; ++k) sum += val(k);
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = 1; o < kScanBlock; o <<= 1) {
+    const unsigned long long v = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0ull;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned long long run = s[threadIdx.x] - sum;
+  for (long k = b; k < e; ++k) {
+    out[k] = (unsigned)run;
+    run += val(k);
+  }
+  if (threadIdx.x == kScanBlock - 1) *total = s[kScanBlock - 1];
+}
+
+// Work items of the column passes, tile by tile (one thread per tile): the
+// tile's triangles in chunks of at most `chunk`, {tile, first id, count, 0}
+// at item_off[tile] onwards.
+__global__ void k_mesh_items(const unsigned *__restrict__ tile_cnt,
+                             const unsigned *__restrict__ tile_off,
+                             const unsigned *__restrict__ item_off, long ntiles, int chunk,
+                             int4 *__restrict__ items) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const unsigned c = tile_cnt[t];
+  unsigned q = item_off[t];
+  for (unsigned f = 0; f < c; f += (unsigned)chunk, ++q) {
+    const unsigned left = c - f;
+    items[q] = make_int4((int)t, (int)(tile_off[t] + f),
+                         (int)(left < (unsigned)chunk ? left : (unsigned)chunk), 0);
+  }
+}
+
+void launch_scan_excl(const unsigned *in, long n, unsigned per, unsigned *out,
+                      unsigned long long *total, hipStream_t st) {
+  k_scan_excl<<<1, kScanBlock, 0, st>>>(in, n, per, out, total);
+}
+
+void launch_mesh_items(const unsigned *tile_cnt, const unsigned *tile_off,
+                       const unsigned *item_off, long ntiles, int4 *items, hipStream_t st) {
+  if (ntiles > 0)
+    k_mesh_items<<<(unsigned)((ntiles + 255) / 256), 256, 0, st>>>(tile_cnt, tile_off, item_off,
+                                                                   ntiles, kMeshChunk, items);
+}
+
 void launch_mesh_check_axes(const double *a, int n, int *bad, hipStream_t st) {
   if (n > 1) k_mesh_check_axes<<<(n + 255) / 256, 256, 0, st>>>(a, n, bad);
 }
