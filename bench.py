@@ -102,15 +102,36 @@ def calibrate_copy(mib: int) -> None:
 
 
 def traffic_from_profile(workload: str):
-    """HBM bytes per launch of the Jacobi kernel from the committed rocprofv3
-    counter summary, if one exists for this workload."""
+    """(HBM bytes per launch of the Jacobi kernel pair, the summary they come
+    from) out of the committed rocprofv3 counter summary, if one exists for
+    this workload."""
     f = ROOT / "profiles" / f"jacobi_{workload}_traffic.json"
+    if not f.exists():
+        return None, None
+    try:
+        d = json.loads(f.read_text())
+        return d.get("bytes_per_launch"), d.get("source")
+    except Exception:
+        return None, None
+
+
+def limiter_from_profile(workload: str):
+    """What bounds J1 and J2, from the committed counter summary of the
+    vector-memory pipeline (profsum pipeline): VALU issue fraction and TA /
+    TD busy fractions per kernel, if a summary exists for this workload."""
+    f = ROOT / "profiles" / f"jacobi_{workload}_pipeline.json"
     if not f.exists():
         return None
     try:
-        return json.loads(f.read_text()).get("bytes_per_launch")
+        d = json.loads(f.read_text())
     except Exception:
         return None
+    out = {"source": d.get("source"), "note": d.get("note")}
+    for k, short in (("k_j1_dev", "j1"), ("k_j2_dev", "j2")):
+        if k in d:
+            out[short] = {q: round(d[k][q], 3) for q in ("valu_issue", "ta_busy", "td_busy",
+                                                        "l1_hit", "l2_hit") if q in d[k]}
+    return out
 
 
 def _nproc() -> int:
@@ -497,7 +518,9 @@ def main():
         "phase_ms_per_step": r["phases"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile(workload),
+                     "traffic": traffic_from_profile(workload)[0],
+                     "traffic_source": traffic_from_profile(workload)[1],
+                     "limiter": limiter_from_profile(workload),
                      "kernel": "k_j1_dev + k_j2_dev (one A p evaluation)",
                      "algorithmic_bytes_per_launch": pair_bytes,
                      "avg_launch_ms": pair_ms,

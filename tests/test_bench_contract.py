@@ -46,6 +46,10 @@ def test_bench_line_fields():
     assert rl["bound"] == "hbm" and rl["unit"] == "GB/s" and rl["peak"] == 8000.0
     assert rl["achieved"] > 0 and abs(rl["frac"] - rl["achieved"] / rl["peak"]) < 1e-12
     assert rl["traffic"] is None or rl["traffic"] > 0
+    assert rl["traffic"] is None or rl["traffic_source"].startswith("profiles/")
+    lim = rl["limiter"]
+    assert lim["source"].startswith("profiles/")
+    assert 0 < lim["j1"]["valu_issue"] <= 1 and 0 < lim["j2"]["td_busy"] <= 1
     assert rl["launches_timed"] > 0
     cb = d["cpu_baseline"]
     assert cb["kind"] == "port" and cb["cores"] == 1 and cb["value"] > 0 and cb["sample"]

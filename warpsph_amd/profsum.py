@@ -3,11 +3,13 @@
     python -m warpsph_amd.profsum kernels <results.db> [--steps N]
     python -m warpsph_amd.profsum dispatches <kernel_trace.csv> [--steps N]
     python -m warpsph_amd.profsum traffic <fetch.db> <write.db> <copy_fetch.db> <copy_write.db> [--copy-bytes B]
+    python -m warpsph_amd.profsum pipeline <results.db> [...] [--json out.json]
 
 ``kernels`` prints per-kernel call counts and durations (the --stats view);
 ``traffic`` prints HBM bytes per dispatch of the Jacobi kernels from
 FETCH_SIZE and WRITE_SIZE runs, scaled by what the same counters read for a
-plain device copy of known size.
+plain device copy of known size; ``pipeline`` the vector-memory pipeline
+figures of the Jacobi kernels (VALU issue, TA/TD busy, cache hit rates).
 """
 from __future__ import annotations
 
@@ -142,6 +144,81 @@ def sq(db: str) -> str:
     return "\n".join(out)
 
 
+PIPELINE = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "GRBM_GUI_ACTIVE", "TA_TA_BUSY",
+            "TA_ADDR_STALLED_BY_TC_CYCLES", "TD_TD_BUSY", "TD_TC_STALL",
+            "TCP_TOTAL_CACHE_ACCESSES", "TCP_TCC_READ_REQ", "TCC_HIT", "TCC_MISS")
+
+
+def pipeline(dbs, kernels=("k_j1_dev", "k_j2_dev"), cus: int = 256, xcds: int = 8) -> dict:
+    """Per kernel, medians over its working dispatches (those with at least
+    half the largest VALU count) of the counters in ``PIPELINE``, gathered
+    from several runs of the same command (one counter group per run), and
+    the derived fractions: VALU issue = 4 cycles per wave instruction over
+    the 4 SIMDs of every CU; TA and TD busy per CU; L1 (TCP) and L2 (TCC)
+    hit rates; L1 cache accesses per vector read instruction. Cycles per
+    dispatch = GRBM_GUI_ACTIVE / number of XCDs."""
+    import statistics
+    runs = []
+    for db in dbs:
+        cur = sqlite3.connect(db).cursor()
+        per = collections.defaultdict(lambda: collections.defaultdict(float))
+        names = {}
+        for d, k, c, v in cur.execute(
+                "select dispatch_id, kernel_name, counter_name, value from counters_collection"):
+            per[d][c] += v
+            names[d] = k.split("(")[0]
+        runs.append((per, names))
+    out = {}
+    for kern in kernels:
+        vals = collections.defaultdict(list)
+        for per, names in runs:
+            mine = {d: cs for d, cs in per.items() if kern in names[d]}
+            if not mine:
+                continue
+            # working dispatches: a group without SQ_INSTS_VALU keeps those with
+            # the run's largest GRBM_GUI_ACTIVE half (the early-return launches
+            # of the device-driven loop last a few microseconds)
+            key = "SQ_INSTS_VALU" if any("SQ_INSTS_VALU" in cs for cs in mine.values()) \
+                else "GRBM_GUI_ACTIVE"
+            top = max(cs.get(key, 0.0) for cs in mine.values())
+            for cs in mine.values():
+                if key in cs and cs[key] >= 0.5 * top:
+                    for c, v in cs.items():
+                        vals[c].append(v)
+        med = {c: statistics.median(v) for c, v in vals.items()}
+        cyc = med.get("GRBM_GUI_ACTIVE", 0.0) / xcds
+        d = dict(counters=med, cycles=cyc)
+        if cyc:
+            if "SQ_INSTS_VALU" in med:
+                d["valu_issue"] = med["SQ_INSTS_VALU"] * 4 / (4 * cus) / cyc
+            for c, name in (("TA_TA_BUSY", "ta_busy"), ("TD_TD_BUSY", "td_busy"),
+                            ("TA_ADDR_STALLED_BY_TC_CYCLES", "ta_stalled_by_tc"),
+                            ("TD_TC_STALL", "td_stalled_by_tc")):
+                if c in med:
+                    d[name] = med[c] / cus / cyc
+        if med.get("TCP_TOTAL_CACHE_ACCESSES"):
+            d["l1_hit"] = 1 - med.get("TCP_TCC_READ_REQ", 0.0) / med["TCP_TOTAL_CACHE_ACCESSES"]
+            if med.get("SQ_INSTS_VMEM_RD"):
+                d["l1_accesses_per_vmem_read"] = med["TCP_TOTAL_CACHE_ACCESSES"] / med["SQ_INSTS_VMEM_RD"]
+        if med.get("TCC_HIT", 0) + med.get("TCC_MISS", 0) > 0:
+            d["l2_hit"] = med["TCC_HIT"] / (med["TCC_HIT"] + med["TCC_MISS"])
+        out[kern] = d
+    return out
+
+
+def pipeline_table(res: dict) -> str:
+    cols = (("valu_issue", "VALU issue"), ("ta_busy", "TA busy"), ("td_busy", "TD busy"),
+            ("ta_stalled_by_tc", "TA<-TC stall"), ("td_stalled_by_tc", "TD<-TC stall"),
+            ("l1_hit", "L1 hit"), ("l2_hit", "L2 hit"),
+            ("l1_accesses_per_vmem_read", "L1 acc/vmem"))
+    out = [f"{'kernel':12s} " + " ".join(f"{h:>12s}" for _, h in cols)]
+    for k, d in res.items():
+        out.append(f"{k:12s} " + " ".join(
+            f"{d[c]:12.3f}" if c in d and c != "l1_accesses_per_vmem_read" else
+            (f"{d[c]:12.1f}" if c in d else f"{'-':>12s}") for c, _ in cols))
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -153,6 +230,9 @@ def main():
     dsp.add_argument("--steps", type=int)
     q = sub.add_parser("sq")
     q.add_argument("db")
+    pl = sub.add_parser("pipeline")
+    pl.add_argument("dbs", nargs="+")
+    pl.add_argument("--json", help="also write the figures to this file")
     t = sub.add_parser("traffic")
     t.add_argument("fetch")
     t.add_argument("write")
@@ -166,6 +246,12 @@ def main():
         print(dispatches(a.csv, a.steps))
     elif a.cmd == "sq":
         print(sq(a.db))
+    elif a.cmd == "pipeline":
+        res = pipeline(a.dbs)
+        print(pipeline_table(res))
+        if a.json:
+            with open(a.json, "w") as fh:
+                json.dump(res, fh, indent=1)
     else:
         print(json.dumps(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes),
                          indent=1))
