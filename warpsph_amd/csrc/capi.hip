@@ -129,7 +129,8 @@ __global__ void k_export_lists(const int *__restrict__ nbr, const int *__restric
   const long o = orig[i];
   const int c = cnt[i];
   cnt_o[o] = c;
-  for (int k = 0; k < c && k < cap_out; ++k) idx_o[o * cap_out + k] = other_orig[nbr[(long)k * n + i]];
+  const int *row = nbr + list_base(i, cap_in);
+  for (int k = 0; k < c && k < cap_out; ++k) idx_o[o * cap_out + k] = other_orig[row[(long)k * kListLanes]];
 }
 __global__ void k_solve_result(DevState *st, int iters, double residual, int converged,
                                int ap_evals) {
@@ -825,8 +826,8 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
     HIP_TRY(dalloc(&s->perm, n));
     HIP_TRY(dalloc(&s->cnt_ff, n));
     HIP_TRY(dalloc(&s->cnt_fs, n));
-    HIP_TRY(dalloc(&s->nbr_ff, (size_t)n * s->P.max_ff));
-    HIP_TRY(dalloc(&s->nbr_fs, (size_t)n * s->P.max_fs));
+    HIP_TRY(dalloc(&s->nbr_ff, list_entries(n, s->P.max_ff)));
+    HIP_TRY(dalloc(&s->nbr_fs, list_entries(n, s->P.max_fs)));
     s->cap_n = n;
   }
   if (s->nscenes > 0 && n > 0) {

@@ -290,9 +290,22 @@ __device__ __forceinline__ int xcd_tile() {
 }
 __device__ __forceinline__ int particle_index() { return xcd_tile() * blockDim.x + threadIdx.x; }
 
-// Visit the neighbours of particle i in list order. Lists are column-major,
-// entry k of particle i at list[k * n + i], so a wave reads slot k of 64
-// consecutive particles in one coalesced access. The neighbours go in batches
+// Neighbour lists are tiled by wave: the entries of the 64 particles
+// 64w .. 64w + 63 form one block of cap rows of 64 entries, row k holding
+// their k-th neighbours. A wave reads row k in one coalesced 256-byte access,
+// and its successive rows are adjacent in memory (a column-major list[k * n
+// + i] puts them n * 4 bytes apart, each row a fresh DRAM page).
+constexpr int kListLanes = 64;
+__host__ __device__ inline long list_base(int i, int cap) {
+  return ((long)(i / kListLanes) * cap) * kListLanes + (i % kListLanes);
+}
+// entries of a list of n particles with cap slots (n rounded up to whole blocks)
+__host__ __device__ inline size_t list_entries(int n, int cap) {
+  return (size_t)((n + kListLanes - 1) / kListLanes) * kListLanes * (size_t)cap;
+}
+
+// Visit the neighbours of particle i in list order: entry k at
+// list[list_base(i, cap) + k * 64]. The neighbours go in batches
 // of kBatch: the batch's gathers are issued together, then the list entries of
 // the next batch, and only then is the batch evaluated, so the list load of
 // one batch overlaps the gathers of the other. The last, partial batch repeats
@@ -300,29 +313,29 @@ __device__ __forceinline__ int particle_index() { return xcd_tile() * blockDim.x
 // body. `load(j, slot)` returns the data of neighbour j (slot = its list
 // position), `body(data)` accumulates in list order.
 template <int kBatch, typename LoadF, typename BodyF>
-__device__ __forceinline__ void for_neighbors(const int *__restrict__ list, int n, int i, int cnt,
+__device__ __forceinline__ void for_neighbors(const int *__restrict__ list, int cap, int i, int cnt,
                                               LoadF load, BodyF body) {
   if (cnt <= 0) return;
-  const int *__restrict__ col = list + i;
+  const int *__restrict__ col = list + list_base(i, cap);
   int jn[kBatch];
 #pragma unroll
-  for (int u = 0; u < kBatch; ++u) jn[u] = col[(long)min(u, cnt - 1) * n];
+  for (int u = 0; u < kBatch; ++u) jn[u] = col[min(u, cnt - 1) * kListLanes];
   int k = 0;
   for (; k + kBatch <= cnt; k += kBatch) {
-    decltype(load(0, 0L)) d[kBatch];
+    decltype(load(0, 0)) d[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) d[u] = load(jn[u], (long)(k + u) * n + i);
+    for (int u = 0; u < kBatch; ++u) d[u] = load(jn[u], k + u);
     if (k + kBatch < cnt) {
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) jn[u] = col[(long)min(k + kBatch + u, cnt - 1) * n];
+      for (int u = 0; u < kBatch; ++u) jn[u] = col[min(k + kBatch + u, cnt - 1) * kListLanes];
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) body(d[u]);
   }
   if (k < cnt) {
-    decltype(load(0, 0L)) d[kBatch];
+    decltype(load(0, 0)) d[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) d[u] = load(jn[u], (long)(k + u) * n + i);
+    for (int u = 0; u < kBatch; ++u) d[u] = load(jn[u], k + u);
 #pragma unroll
     for (int u = 0; u < kBatch; ++u)
       if (k + u < cnt) body(d[u]);

@@ -64,8 +64,8 @@ __global__ __launch_bounds__(64 * kLearnWaves, 4) void k_learned_start(Bufs b, S
     const float4 *__restrict__ pos = b.pos;
     const float4 *__restrict__ seed = b.seed;
     for_neighbors<4>(
-        b.nbr_ff, n, i, b.cnt_ff[i],
-        [&](int j, long) { return PosSeed{ld_at(pos, j), ld_at(seed, j)}; },
+        b.nbr_ff, b.cap_ff, i, b.cnt_ff[i],
+        [&](int j, int) { return PosSeed{ld_at(pos, j), ld_at(seed, j)}; },
         [&](const PosSeed &v) {
           const float w = spline_w(wtab, c.nw, c.w_scale, c.h,
                                    dist3(p.x, p.y, p.z, v.pos.x, v.pos.y, v.pos.z));

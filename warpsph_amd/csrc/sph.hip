@@ -255,7 +255,7 @@ static_assert((kListRing & (kListRing - 1)) == 0, "ring rows: a power of two");
 struct ListWriter {
   int *ring;      // [kListRing][64] of this wave
   int *list;
-  long n;
+  long base;      // list_base(i, cap): entry k at list[base + k * 64]
   int i, lane, cap;
   bool live;
   int cnt = 0, direct = 0x7fffffff, flushed = 0;
@@ -265,7 +265,7 @@ struct ListWriter {
         ring[((unsigned)cnt & (kListRing - 1)) * 64 + lane] = j;
       } else {
         if (direct == 0x7fffffff) direct = cnt;
-        list[(long)cnt * n + i] = j;
+        list[base + (long)cnt * kListLanes] = j;
       }
     }
     ++cnt;
@@ -284,7 +284,8 @@ struct ListWriter {
   }
   __device__ void flush_to(int upto) {  // wave-uniform
     for (int k = flushed; k < upto; ++k)
-      if (live && k < cnt && k < cap && k < direct) list[(long)k * n + i] = ring[((unsigned)k & (kListRing - 1)) * 64 + lane];
+      if (live && k < cnt && k < cap && k < direct)
+        list[base + (long)k * kListLanes] = ring[((unsigned)k & (kListRing - 1)) * 64 + lane];
     flushed = upto > flushed ? upto : flushed;
   }
   __device__ void row_done() {  // wave-uniform: rows every lane has completed
@@ -366,7 +367,7 @@ __global__ void k_lists(Bufs b, SphConst c) {
   const int n = b.n;
   const float4 p = live ? b.pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const int scene = b.scenes && live ? b.sid[i] : 0;
-  ListWriter wf{ring, b.nbr_ff, n, i, (int)(threadIdx.x & 63), b.cap_ff, live};
+  ListWriter wf{ring, b.nbr_ff, list_base(i, b.cap_ff), i, (int)(threadIdx.x & 63), b.cap_ff, live};
   const GridRef fg = fluid_grid(b, c, scene);
   for_query_rows(
       p.x, p.y, p.z, live, c, fg.dims, fg.start, fg.count,
@@ -376,7 +377,7 @@ __global__ void k_lists(Bufs b, SphConst c) {
       },
       [&]() { wf.row_done(); });
   wf.finish();
-  ListWriter wb{ring, b.nbr_fs, n, i, (int)(threadIdx.x & 63), b.cap_fs, live};
+  ListWriter wb{ring, b.nbr_fs, list_base(i, b.cap_fs), i, (int)(threadIdx.x & 63), b.cap_fs, live};
   const GridRef bg = boundary_grid(b, c, scene);
   if (b.nb > 0 && __ballot(live && near_boundary(b, c, bg, p))) {
     for_query_rows(
@@ -470,7 +471,7 @@ __global__ void k_density(Bufs b, SphConst c, int fuse_vadv) {
   float t1 = 0.0f, t2 = 0.0f;
   float dx1 = 0.0f, dy1 = 0.0f, dz1 = 0.0f, dx2 = 0.0f, dy2 = 0.0f, dz2 = 0.0f;
   for_neighbors<kBatch>(
-      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return ld_at(pos, j); },
+      b.nbr_ff, b.cap_ff, i, b.cnt_ff[i], [&](int j, int) { return ld_at(pos, j); },
       [&](const float4 &o) {
         const float3 r = sub3(o, p);
         const float len = sqrt_rn(r.x * r.x + r.y * r.y + r.z * r.z);
@@ -481,7 +482,7 @@ __global__ void k_density(Bufs b, SphConst c, int fuse_vadv) {
         dz1 = dz1 - gw.z;
       });
   for_neighbors<kBatch>(
-      b.nbr_fs, b.n, i, b.cnt_fs[i], [&](int j, long) { return ld_at(bpos, j); },
+      b.nbr_fs, b.cap_fs, i, b.cnt_fs[i], [&](int j, int) { return ld_at(bpos, j); },
       [&](const float4 &o) {
         const float3 r = sub3(o, p);
         const float len = sqrt_rn(r.x * r.x + r.y * r.y + r.z * r.z);
@@ -518,7 +519,7 @@ __global__ void k_vadv(Bufs b, SphConst c) {
   if (c.vis_mu != 0.0f) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
     for_neighbors<kBatch>(
-        b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{ld_at(pos, j), ld_at(vel, j)}; },
+        b.nbr_ff, b.cap_ff, i, nf, [&](int j, int) { return F4x2{ld_at(pos, j), ld_at(vel, j)}; },
         [&](const F4x2 &v) {
           const float4 o = v.a, vj = v.b;
           const float3 xij = sub3(o, p);
@@ -532,7 +533,7 @@ __global__ void k_vadv(Bufs b, SphConst c) {
           az = az - gw.z * dd / o.w / r2;
         });
     for_neighbors<kBatch>(
-        b.nbr_fs, b.n, i, ns, [&](int j, long) { return ld_at(bpos, j); },
+        b.nbr_fs, b.cap_fs, i, ns, [&](int j, int) { return ld_at(bpos, j); },
         [&](const float4 &va_) {
           const F4x2 v{va_, va_};
           const float4 o = v.a;
@@ -556,7 +557,7 @@ __global__ void k_vadv(Bufs b, SphConst c) {
     // XSPH: v_adv += xsph * sum_j (v_j - v_i) * ((m / rho_j) * W_ij)
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     for_neighbors<kBatch>(
-        b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{ld_at(pos, j), ld_at(vel, j)}; },
+        b.nbr_ff, b.cap_ff, i, nf, [&](int j, int) { return F4x2{ld_at(pos, j), ld_at(vel, j)}; },
         [&](const F4x2 &v) {
           const float4 o = v.a, vj = v.b;
           const float coef =
@@ -597,7 +598,7 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
   const float dix = di.x, diy = di.y, diz = di.z;
   float tr1 = 0.0f, tr2 = 0.0f, ta1 = 0.0f, ta2 = 0.0f;
   for_neighbors<kBatch>(
-      b.nbr_ff, b.n, i, nf, [&](int j, long) { return F4x2{ld_at(pos, j), ld_at(vadv, j)}; },
+      b.nbr_ff, b.cap_ff, i, nf, [&](int j, int) { return F4x2{ld_at(pos, j), ld_at(vadv, j)}; },
       [&](const F4x2 &v) {
         const float3 gw = GW_OF(T, sub3(v.a, p));
         tr1 += dot3(make_float3(va.x - v.b.x, va.y - v.b.y, va.z - v.b.z), gw);
@@ -607,7 +608,7 @@ __global__ void k_assemble(Bufs b, SphConst c, int start_mode) {
         ta1 += dot3(dd, gw);
       });
   for_neighbors<kBatch>(
-      b.nbr_fs, b.n, i, ns, [&](int j, long) { return F4x2{ld_at(bpos, j), ld_at(bvel, j)}; },
+      b.nbr_fs, b.cap_fs, i, ns, [&](int j, int) { return F4x2{ld_at(bpos, j), ld_at(bvel, j)}; },
       [&](const F4x2 &v) {
         const float4 o = v.a, vb = v.b;
         const float3 gw = GW_OF(T, sub3(o, p));
@@ -647,7 +648,7 @@ __device__ __forceinline__ void j1_particle(const Bufs &b, const SphConst &c, co
   const float yi = 1.0f / ri2;
   float t1 = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
   for_neighbors<kBatchJ1>(
-      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return F4x2{ld_at(pos, j), ld_at(dp, j)}; },
+      b.nbr_ff, b.cap_ff, i, b.cnt_ff[i], [&](int j, int) { return F4x2{ld_at(pos, j), ld_at(dp, j)}; },
       [&](const F4x2 &v) {
         const float4 o = v.a, dj = v.b;
         const float3 gw = GW_OF(T, sub3(o, p));
@@ -679,13 +680,13 @@ __device__ __forceinline__ void j2_particle(const Bufs &b, const SphConst &c, co
   const float4 *__restrict__ bpos = b.bpos;
   float t1 = 0.0f, t2 = 0.0f;
   for_neighbors<kBatchJ2>(
-      b.nbr_ff, b.n, i, b.cnt_ff[i], [&](int j, long) { return F4x2{ld_at(pos, j), ld_at(sap, j)}; },
+      b.nbr_ff, b.cap_ff, i, b.cnt_ff[i], [&](int j, int) { return F4x2{ld_at(pos, j), ld_at(sap, j)}; },
       [&](const F4x2 &v) {
         const float3 gw = GW_OF(T, sub3(v.a, p));
         t1 += dot3(make_float3(si.x - v.b.x, si.y - v.b.y, si.z - v.b.z), gw);
       });
   for_neighbors<kBatchJ2>(
-      b.nbr_fs, b.n, i, b.cnt_fs[i], [&](int j, long) { return ld_at(bpos, j); },
+      b.nbr_fs, b.cap_fs, i, b.cnt_fs[i], [&](int j, int) { return ld_at(bpos, j); },
       [&](const float4 &va_) {
         const F4x2 v{va_, va_};
         const float3 gw = GW_OF(T, sub3(v.a, p));
