@@ -75,7 +75,7 @@ struct Bufs {
   double *sc_res;       // residual at the scene's exit
   // fluid grid
   const unsigned *fstart, *fcount;
-  // neighbour lists, column-major: list[k * n + i]
+  // neighbour lists, tiled by wave: entry k of i at list[list_base(i, cap) + k * 64]
   int *nbr_ff, *cnt_ff, *nbr_fs, *cnt_fs;
   int cap_ff, cap_fs;
   // boundary (sorted by its own grid)

@@ -4,8 +4,9 @@
 // particle's neighbours in the reference's enumeration order and every
 // expression keeps the reference's operation order (IISPH.py:23-699), so the
 // float32 results equal those of the CPU restatement exactly. Neighbour
-// lists are column-major (list[k * n + i]) so that lane i of a wave reads
-// slot k of 64 consecutive particles in one coalesced access. The kernel
+// lists are tiled by wave (entry k of particle i at list[list_base(i, cap) +
+// k * 64]) so that a wave reads slot k of its 64 particles in one coalesced
+// access. The kernel
 // tables sit in LDS when they fit (sizes up to kMaxLdsTable floats).
 #include "kernels.h"
 
@@ -242,7 +243,7 @@ __device__ __forceinline__ void for_query_rows(float px, float py, float pz, boo
 }
 
 // Neighbour list writer of one wave. Entry k of lane l belongs at
-// list[k * n + i_l]; the lanes find their entries at different times, and
+// list[list_base(i_l, cap) + k * 64]; the lanes find their entries at different times, and
 // storing each as found fills a 128-byte line of slot k from ~32 separate
 // partial stores, between which the line is evicted and written back again
 // under the candidate-read traffic. Entries are therefore kept in a ring of
