@@ -730,7 +730,7 @@ int wsph_set_boundary(wsph_solver *s, const float *xb, const float *vb, int nb) 
   dfree(s->borig);
   dfree(s->bnear);
   s->nb = nb;
-  HIP_TRY(dalloc(&s->bpos, nb));
+  HIP_TRY(dalloc(&s->bpos, nb + kPointPad));
   HIP_TRY(dalloc(&s->bvel, nb));
   HIP_TRY(dalloc(&s->borig, nb));
   HIP_TRY(dalloc(&s->bnear, s->bg.ncell));
@@ -801,7 +801,7 @@ int wsph_set_fluid(wsph_solver *s, const float *x, const float *v, int n) {
   if (n != s->cap_n || !s->pos[0]) {  // (a first call with n = 0 allocates too)
     free_fluid(s);
     for (int k = 0; k < 2; ++k) {
-      HIP_TRY(dalloc(&s->pos[k], n));
+      HIP_TRY(dalloc(&s->pos[k], n + kPointPad));
       HIP_TRY(dalloc(&s->vel[k], n));
       HIP_TRY(dalloc(&s->pprev[k], n));
       HIP_TRY(dalloc(&s->orig[k], n));

@@ -304,6 +304,11 @@ __host__ __device__ inline size_t list_entries(int n, int cap) {
   return (size_t)((n + kListLanes - 1) / kListLanes) * kListLanes * (size_t)cap;
 }
 
+// The fluid and boundary position arrays are allocated with kPointPad entries
+// past their last particle: the list builder loads candidate batches ahead of
+// the span end, and masks what lies beyond it.
+constexpr int kPointPad = 8;
+
 // Visit the neighbours of particle i in list order: entry k at
 // list[list_base(i, cap) + k * 64]. The neighbours go in batches
 // of kBatch: the batch's gathers are issued together, then the list entries of
@@ -365,5 +370,4 @@ __device__ __forceinline__ int wave_max_i32(int v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
   return v;
 }
-
 }  // namespace wsph

@@ -253,94 +253,127 @@ __device__ __forceinline__ void for_query_rows(float px, float py, float pz, boo
 // flushed rows stores its later entries directly.
 constexpr int kListRing = 32;  // a power of two
 static_assert((kListRing & (kListRing - 1)) == 0, "ring rows: a power of two");
+constexpr unsigned kRingRowMask = (kListRing - 1) * 256u;  // byte offset of a ring row
+typedef __attribute__((address_space(3))) int lds_int;  // an int in shared memory
+#ifndef WSPH_LIST_BATCH
+#define WSPH_LIST_BATCH 4
+#endif
+constexpr int kListBatch = WSPH_LIST_BATCH;  // candidates loaded together per lane
+static_assert(kListBatch <= kPointPad, "a batch reads at most kPointPad places past a span");
 struct ListWriter {
   int *ring;      // [kListRing][64] of this wave
   int *list;
   long base;      // list_base(i, cap): entry k at list[base + k * 64]
   int i, lane, cap;
   bool live;
-  int cnt = 0, direct = 0x7fffffff, flushed = 0;
+  // Entries found so far times 256: entry k's ring slot lies at byte
+  // (k * 256) mod (kListRing * 256) + lane * 4 of the wave's ring, so the
+  // running count is also the slot's row offset.
+  unsigned cb = 0;
+  int direct = 0x7fffffff, flushed = 0;
+  // put_batch may store entries up to lim - 1 (ring rows not yet flushed and
+  // below the capacity), times 256; 0 once entries go straight to the list.
+  unsigned limb = 0;
+  // shared-memory address of this lane's slot in ring row 0; put_batch ORs
+  // the row offset into it, which needs the wave's ring at a multiple of
+  // kListRing * 256 bytes (else put_batch is not used)
+  unsigned lane_at = 0;
+  __device__ int cnt() const { return (int)(cb >> 8); }
+  __device__ void init() {
+    const unsigned at = (unsigned)(size_t)(lds_int *)ring;
+    lane_at = at + (unsigned)lane * 4u;
+    limb = (at & (kListRing * 256u - 1u)) == 0u ? (unsigned)(kListRing < cap ? kListRing : cap) << 8
+                                                : 0u;
+  }
   __device__ void put(int j) {
-    if (cnt < cap) {
-      if (direct == 0x7fffffff && cnt - flushed < kListRing) {
-        ring[((unsigned)cnt & (kListRing - 1)) * 64 + lane] = j;
+    const int k = cnt();
+    if (k < cap) {
+      if (direct == 0x7fffffff && k - flushed < kListRing) {
+        ring[((unsigned)k & (kListRing - 1)) * 64 + lane] = j;
       } else {
-        if (direct == 0x7fffffff) direct = cnt;
-        list[base + (long)cnt * kListLanes] = j;
+        if (direct == 0x7fffffff) direct = k;
+        list[base + (long)k * kListLanes] = j;
+        limb = 0;
       }
     }
-    ++cnt;
+    cb += 256u;
   }
-  // Room for k more entries in the ring without reaching an unflushed row or
-  // the capacity: then put_fast may be used k times.
-  __device__ bool room(int k) const {
-    return direct == 0x7fffffff && cnt + k - flushed <= kListRing && cnt + k <= cap;
-  }
-  // Candidate j with its test result, without a branch: j goes to the next
-  // free ring slot, which only a hit keeps (a miss is overwritten by the next
-  // candidate, and slots at or past cnt are never flushed).
-  __device__ void put_fast(int j, bool hit) {
-    ring[((unsigned)cnt & (kListRing - 1)) * 64 + lane] = j;
-    cnt += hit ? 1 : 0;
+  // Room for k more entries in the ring: then put_batch may be used.
+  __device__ bool room(int k) const { return cb + ((unsigned)k << 8) <= limb; }
+  // Candidates j, j + 1, ... with their test results, without a branch: each
+  // goes to the next free ring slot, which only a hit keeps (a miss is
+  // overwritten by the next candidate, and slots at or past the count are
+  // never flushed).
+  template <int K>
+  __device__ void put_batch(const int (&val)[K], const bool (&hit)[K]) {
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      *(lds_int *)(size_t)((cb & kRingRowMask) | lane_at) = val[u];
+      cb += hit[u] ? 256u : 0u;
+    }
   }
   __device__ void flush_to(int upto) {  // wave-uniform
+    const int n = cnt();
     for (int k = flushed; k < upto; ++k)
-      if (live && k < cnt && k < cap && k < direct)
+      if (live && k < n && k < cap && k < direct)
         list[base + (long)k * kListLanes] = ring[((unsigned)k & (kListRing - 1)) * 64 + lane];
     flushed = upto > flushed ? upto : flushed;
+    if (direct == 0x7fffffff && limb != 0u)
+      limb = (unsigned)(flushed + kListRing < cap ? flushed + kListRing : cap) << 8;
   }
   __device__ void row_done() {  // wave-uniform: rows every lane has completed
-    int m = live ? cnt : 0x7fffffff;
+    int m = live ? cnt() : 0x7fffffff;
     for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
     m = m < cap ? m : cap;
     if (m > flushed) flush_to(m);
   }
   __device__ void finish() {
-    const int m = wave_max_i32(live ? (cnt < cap ? cnt : cap) : 0);
+    const int n = cnt();
+    const int m = wave_max_i32(live ? (n < cap ? n : cap) : 0);
     flush_to(m);
   }
 };
 
-// Candidates [s0, e0) of one span tested against p in batches of kBatch
-// loads, each result written with put_fast (no branch per candidate) while
-// the ring has room, else with put for the hits; the last, partial batch
-// loads candidate e0 - 1 in its unused places and masks them out. Hits go to
-// the writer in ascending candidate order (the order of a one-by-one scan).
-// self: the index to skip (the particle itself), -1 for none.
-__device__ __forceinline__ void emit_batch(ListWriter &w, int j, const bool (&hit)[kBatch]) {
-  if (__all(w.room(kBatch))) {  // one wave-uniform branch in the usual case
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) w.put_fast(j + u, hit[u]);
+// Hits of one batch of candidates (indices val, in scan order) go to the
+// writer in that order (the order of a one-by-one scan): through put_batch
+// while every lane's ring has room, else one by one through put.
+__device__ __forceinline__ void emit_batch(ListWriter &w, const int (&val)[kListBatch],
+                                           const bool (&hit)[kListBatch]) {
+  if (__all(w.room(kListBatch))) {  // one wave-uniform branch in the usual case
+    w.put_batch(val, hit);
   } else {
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u)
-      if (hit[u]) w.put(j + u);
+    for (int u = 0; u < kListBatch; ++u)
+      if (hit[u]) w.put(val[u]);
   }
 }
 
+// Candidates j, j + 1, ... of a batch tested against p; places at or past e0
+// (the padding after the array, or the next span's particles) are masked out.
+// self: the index to skip (the particle itself), -1 for none.
+__device__ __forceinline__ void test_batch(ListWriter &w, const float4 (&o)[kListBatch], int j,
+                                           int e0, const float4 &p, float r2_lt, int self) {
+  bool hit[kListBatch];
+  int val[kListBatch];
+#pragma unroll
+  for (int u = 0; u < kListBatch; ++u) {
+    val[u] = j + u;
+    hit[u] = (j + u < e0) & (j + u != self) &
+             (dist2(p.x, p.y, p.z, o[u].x, o[u].y, o[u].z) < r2_lt);
+  }
+  emit_batch(w, val, hit);
+}
+
+// Candidates [s0, e0) of one span, kListBatch at a time. The last batch may
+// read up to kListBatch - 1 places past e0 (at most kPointPad past the
+// array's last particle) and masks them out.
 __device__ __forceinline__ void span_hits(const float4 *__restrict__ pts, int s0, int e0,
                                           const float4 &p, float r2_lt, int self, ListWriter &w) {
-  int j = s0;
-  for (; j + kBatch <= e0; j += kBatch) {
-    float4 o[kBatch];
+  for (int j = s0; j < e0; j += kListBatch) {
+    float4 o[kListBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) o[u] = pts[j + u];
-    bool hit[kBatch];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u)
-      hit[u] = (j + u != self) & (dist2(p.x, p.y, p.z, o[u].x, o[u].y, o[u].z) < r2_lt);
-    emit_batch(w, j, hit);
-  }
-  if (j < e0) {
-    float4 o[kBatch];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) o[u] = pts[min(j + u, e0 - 1)];
-    bool hit[kBatch];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u)
-      hit[u] = (j + u < e0) & (j + u != self) &
-               (dist2(p.x, p.y, p.z, o[u].x, o[u].y, o[u].z) < r2_lt);
-    emit_batch(w, j, hit);
+    for (int u = 0; u < kListBatch; ++u) o[u] = pts[j + u];
+    test_batch(w, o, j, e0, p, r2_lt, self);
   }
 }
 
@@ -363,12 +396,13 @@ __device__ __forceinline__ bool near_boundary(const Bufs &b, const SphConst &c, 
 
 __global__ void k_lists(Bufs b, SphConst c) {
   int *ring = reinterpret_cast<int *>(g_smem) + (threadIdx.x >> 6) * (kListRing * 64);
+  const int lane = (int)(threadIdx.x & 63);
   const int i = particle_index();
   const bool live = i < b.n;
-  const int n = b.n;
   const float4 p = live ? b.pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const int scene = b.scenes && live ? b.sid[i] : 0;
-  ListWriter wf{ring, b.nbr_ff, list_base(i, b.cap_ff), i, (int)(threadIdx.x & 63), b.cap_ff, live};
+  ListWriter wf{ring, b.nbr_ff, list_base(i, b.cap_ff), i, lane, b.cap_ff, live};
+  wf.init();
   const GridRef fg = fluid_grid(b, c, scene);
   for_query_rows(
       p.x, p.y, p.z, live, c, fg.dims, fg.start, fg.count,
@@ -378,7 +412,8 @@ __global__ void k_lists(Bufs b, SphConst c) {
       },
       [&]() { wf.row_done(); });
   wf.finish();
-  ListWriter wb{ring, b.nbr_fs, list_base(i, b.cap_fs), i, (int)(threadIdx.x & 63), b.cap_fs, live};
+  ListWriter wb{ring, b.nbr_fs, list_base(i, b.cap_fs), i, lane, b.cap_fs, live};
+  wb.init();
   const GridRef bg = boundary_grid(b, c, scene);
   if (b.nb > 0 && __ballot(live && near_boundary(b, c, bg, p))) {
     for_query_rows(
@@ -390,13 +425,13 @@ __global__ void k_lists(Bufs b, SphConst c) {
         [&]() { wb.row_done(); });
     wb.finish();
   }
-  const int nf = wf.cnt, ns = wb.cnt;
+  const int nf = wf.cnt(), ns = wb.cnt();
   if (live) {
     b.cnt_ff[i] = nf < b.cap_ff ? nf : b.cap_ff;
     b.cnt_fs[i] = ns < b.cap_fs ? ns : b.cap_fs;
   }
   const int mf = wave_max_i32(nf), ms = wave_max_i32(ns);
-  if ((threadIdx.x & 63) == 0) {
+  if (lane == 0) {
     atomicMax(&b.st->max_ff, mf);
     atomicMax(&b.st->max_fs, ms);
     if (mf > b.cap_ff) atomicOr(&b.st->status, (int)ST_FF_OVERFLOW);
@@ -1477,9 +1512,12 @@ void launch_mark_bnear(const Bufs &b, const SphConst &c, unsigned *bnear, hipStr
 }
 
 // k_lists runs in blocks of kListsBlock threads whatever block the other
-// passes use: its LDS ring (8 KB per wave) and registers then admit 5 waves
-// per SIMD, against 4 in 512-thread blocks (two 64 KB blocks per CU).
-// Measured at 1M particles: 442 us at 128, 449 at 256, 454 at 512.
+// passes use: its LDS ring (8 KB per wave) and 100 registers then admit 4
+// waves per SIMD. It is bound by the texture data path (about two cycles per
+// cache line a wave's gather touches; its lanes sit in a dozen or more query
+// cells, so a gather touches ~19 lines), not by its VALU work or occupancy:
+// 4 or 5 waves per SIMD, one or two batches in flight all measured 422-425 us
+// at 1M particles.
 constexpr int kListsBlock = 128;
 
 void launch_lists(const Bufs &b, const SphConst &c, int /*other passes' block*/, hipStream_t s) {

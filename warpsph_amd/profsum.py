@@ -233,6 +233,8 @@ def main():
     pl = sub.add_parser("pipeline")
     pl.add_argument("dbs", nargs="+")
     pl.add_argument("--json", help="also write the figures to this file")
+    pl.add_argument("--kernels", nargs="+", default=["k_j1_dev", "k_j2_dev"],
+                    help="kernel name substrings to summarise")
     t = sub.add_parser("traffic")
     t.add_argument("fetch")
     t.add_argument("write")
@@ -247,7 +249,7 @@ def main():
     elif a.cmd == "sq":
         print(sq(a.db))
     elif a.cmd == "pipeline":
-        res = pipeline(a.dbs)
+        res = pipeline(a.dbs, tuple(a.kernels))
         print(pipeline_table(res))
         if a.json:
             with open(a.json, "w") as fh:
