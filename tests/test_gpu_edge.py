@@ -1,4 +1,5 @@
 """Edge cases of the HIP path: no fluid, isolated particles, no boundary,
+the neighbour-list writer past its ring and at exact capacities,
 neighbour-list capacity overflow (an error, where the reference overruns its
 n*60 slots), a batch containing an empty scene."""
 import sys
@@ -6,7 +7,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import assert_same
+from helpers import assert_same, oracle_setup
 from warpsph_amd import config, scene
 from warpsph_amd import _native as N
 
@@ -77,6 +78,60 @@ def test_no_boundary_free_fall_matches_oracle(orc):
     assert_same("x", s.x.cpu().numpy(), o.x)
     assert_same("p", s.p.cpu().numpy(), info["p"])
     assert s.fs_neighbor_num.cpu().numpy().max() == 0
+
+
+def _lists_match_oracle(orc, cfg, sc, s):
+    """Fluid and boundary lists equal the oracle's; returns the fluid counts."""
+    _, P, _, _ = oracle_setup(cfg, sc)
+    counts = []
+    for which, (xs, excl, dims) in enumerate([(sc.x, True, P.grid_dims),
+                                              (sc.xb, False, P.bgrid_dims)]):
+        cnt_o, idx_o, _ = orc.neighbors(sc.x, xs, cfg.h, tuple(dims), excl, cap=64)
+        cnt, idx = s.neighbors(which, cap=64)
+        cnt, idx = cnt.cpu().numpy(), idx.cpu().numpy()
+        assert_same(f"count[{which}]", cnt, cnt_o)
+        for i in range(sc.n):
+            assert idx[i, :cnt[i]].tolist() == idx_o[i, :cnt_o[i]].tolist(), (which, i)
+        counts.append(cnt_o)
+    return counts[0]
+
+
+def test_lists_past_the_ring(orc):
+    """One wave whose lanes have 39 and 0 neighbours: a 40-particle cluster
+    (every pair closer than h) and 24 isolated particles. The wave's lowest
+    count stays 0, so its flushed list rows stay at 0 and the cluster's lanes
+    store their entries 32 onwards straight to the list instead of through
+    the 32-row ring; the lists must still equal the oracle's."""
+    _announce("edge: neighbour lists past the writer's ring")
+    cfg = config.dambreak_8k()
+    base = scene.build(cfg)
+    rng = np.random.default_rng(5)
+    cluster = 0.5 + rng.uniform(-0.01, 0.01, size=(40, 3))  # diagonal 0.035 < h = 0.04
+    lone = np.array([[0.1 * k, y, z] for (y, z) in ((0.2, 0.2), (0.2, 0.8), (0.8, 0.2))
+                     for k in range(1, 9)])
+    sc = _scene(np.concatenate([cluster, lone]), base.xb)
+    s = _solver(cfg, sc)
+    s.neighbor_search()
+    cnt = _lists_match_oracle(orc, cfg, sc, s)
+    assert cnt[:40].min() == 39 and cnt[40:].max() == 0
+
+
+def test_lists_at_their_largest_counts(orc):
+    """Capacities at the scene's largest neighbour counts (rounded up to the
+    library's multiple of 4): the writer's ring limit then comes from the
+    capacity (below the ring's 32 rows for the boundary lists), and the lists
+    still equal the oracle's."""
+    cfg = config.dambreak_8k()
+    sc = scene.build(cfg)
+    _, P, _, _ = oracle_setup(cfg, sc)
+    mf = int(orc.neighbors(sc.x, sc.x, cfg.h, tuple(P.grid_dims), True, cap=64)[0].max())
+    mb = int(orc.neighbors(sc.x, sc.xb, cfg.h, tuple(P.bgrid_dims), False, cap=64)[0].max())
+    _announce(f"edge: neighbour lists at capacity {mf}/{mb}")
+    assert mb < 32 <= mf
+    cfg = cfg.replace(max_ff=-(-mf // 4) * 4, max_fs=-(-mb // 4) * 4)
+    s = _solver(cfg, sc)
+    s.neighbor_search()
+    _lists_match_oracle(orc, cfg, sc, s)
 
 
 def test_neighbour_capacity_overflow_is_an_error():
