@@ -49,7 +49,7 @@ def test_picks_fastest_and_caches(tmp_path, no_gpu):
 
     cache = tmp_path / "tune.json"
     r = autotune.tune(make, config.dambreak_8k(), key="w", cands=cands, steps=3, warmup=1,
-                      cache=cache)
+                      cache=cache, finalists=0)
     assert r["shape"]["block"] == 256 and not r["cached"]
     assert len(r["table"]) == 3 and made == [128, 256, 512]
     data = json.loads(cache.read_text())
@@ -60,8 +60,33 @@ def test_picks_fastest_and_caches(tmp_path, no_gpu):
     assert r2["cached"] and r2["shape"] == r["shape"] and made == []
     # force re-measures
     r3 = autotune.tune(make, config.dambreak_8k(), key="w", cands=cands, steps=2, warmup=0,
-                       cache=cache, force=True)
+                       cache=cache, force=True, finalists=0)
     assert not r3["cached"] and made == [128, 256, 512]
+
+
+def test_finalists_overrule_a_misleading_first_timing(tmp_path, no_gpu):
+    """The first solver built for block 128 is fast, later ones slow (a
+    first timing that does not hold); block 256 is steadily faster. The
+    first pass ranks 128 first; the finalists' re-timings, two rounds in
+    alternating order, pick 256."""
+    cands = [dict(block=b, cell_layout=1, jacobi_mode=2, chunk_iters=1, coop_blocks_per_cu=0,
+                  graph_iters=0) for b in (128, 256, 512)]
+    made = []
+
+    def make(cfg):
+        made.append(cfg.block)
+        first128 = cfg.block == 128 and made.count(128) == 1
+        cost = 0.002 if first128 else (0.004 if cfg.block == 256 else 0.008)
+        return FakeSolver(cfg, lambda c: cost)
+
+    r = autotune.tune(make, config.dambreak_8k(), key="w", cands=cands, steps=3, warmup=1,
+                      cache=None, finalists=2, rounds=2)
+    assert [t["shape"]["block"] for t in sorted(r["table"], key=lambda t: t["ms_per_step"])] == \
+        [128, 256, 512]
+    assert made == [128, 256, 512, 128, 256, 256, 128]
+    assert r["shape"]["block"] == 256
+    assert [f["shape"]["block"] for f in r["final"]] == [128, 256]
+    assert all(len(f["ms"]) == 2 for f in r["final"])
 
 
 def test_apply():

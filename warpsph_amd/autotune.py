@@ -7,7 +7,8 @@ hipGraph holding 2 or 4 iterations, or the persistent cooperative kernel at 1
 or 2 blocks per CU) and cell layout (Morton or linear). Every candidate computes the same numbers
 (the neighbour order and the float operations do not depend on the shape),
 so the candidates run identical work from the same initial state and only
-their wall time differs. Results are cached per device and workload in a
+their wall time differs; the fastest few are timed again before one is
+chosen. Results are cached per device and workload in a
 JSON file (default ``~/.cache/warpsph_amd/autotune.json``).
 """
 from __future__ import annotations
@@ -50,13 +51,35 @@ def _load(cache: Path) -> dict:
         return {}
 
 
+def _time_shape(make, cfg: cfgmod.SolverConfig, shape: dict, warmup: int, steps: int) -> float:
+    """Milliseconds per step of ``steps`` steps after ``warmup``, on a fresh solver."""
+    import torch
+    s = make(cfg.replace(**shape))
+    try:
+        for _ in range(warmup):
+            s.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e3
+    finally:
+        s.close()
+
+
 def tune(make: Callable[[cfgmod.SolverConfig], object], cfg: cfgmod.SolverConfig, key: str,
          cands: Optional[List[dict]] = None, steps: int = 10, warmup: int = 3,
-         cache: Optional[Path] = DEFAULT_CACHE, force: bool = False, log=None) -> Dict:
+         cache: Optional[Path] = DEFAULT_CACHE, force: bool = False, log=None,
+         finalists: int = 4, rounds: int = 2) -> Dict:
     """Best shape for ``make(cfg.replace(**shape))`` by wall time of ``steps``
-    steps after ``warmup``; returns {"shape": ..., "ms_per_step": ..., "table": [...]}.
-    ``make`` builds a solver with ``step()`` and ``close()``."""
-    import torch
+    steps after ``warmup``; returns {"shape": ..., "ms_per_step": ..., "table": [...],
+    "final": [...]}. ``make`` builds a solver with ``step()`` and ``close()``.
+
+    Every candidate is timed once, in order; the ``finalists`` fastest are then
+    timed ``rounds`` more times each, in alternating order, and the one with
+    the lowest of those timings wins. One timing per shape can differ from the
+    next by a few percent, more than the gap between the leading shapes."""
     full_key = f"{_device_name()}|{key}"
     if cache is not None and not force:
         hit = _load(Path(cache)).get(full_key)
@@ -64,23 +87,24 @@ def tune(make: Callable[[cfgmod.SolverConfig], object], cfg: cfgmod.SolverConfig
             return dict(hit, cached=True)
     table = []
     for shape in cands if cands is not None else candidates():
-        s = make(cfg.replace(**shape))
-        try:
-            for _ in range(warmup):
-                s.step()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                s.step()
-            torch.cuda.synchronize()
-            ms = (time.perf_counter() - t0) / steps * 1e3
-        finally:
-            s.close()
+        ms = _time_shape(make, cfg, shape, warmup, steps)
         table.append(dict(shape=shape, ms_per_step=ms))
         if log:
             log(f"autotune {key}: {shape} {ms:.3f} ms/step")
-    best = min(table, key=lambda r: r["ms_per_step"])
-    out = dict(shape=best["shape"], ms_per_step=best["ms_per_step"], table=table, steps=steps,
+    top = [r["shape"] for r in sorted(table, key=lambda r: r["ms_per_step"])[:max(finalists, 1)]]
+    final = [dict(shape=sh, ms=[]) for sh in top]
+    if finalists > 0 and rounds > 0 and len(top) > 1:
+        for rnd in range(rounds):
+            for f in (final if rnd % 2 == 0 else final[::-1]):
+                f["ms"].append(_time_shape(make, cfg, f["shape"], warmup, steps))
+                if log:
+                    log(f"autotune {key} finalist: {f['shape']} {f['ms'][-1]:.3f} ms/step")
+        best = min(final, key=lambda f: min(f["ms"]))
+        best_shape, best_ms = best["shape"], min(best["ms"])
+    else:
+        best_shape = top[0]
+        best_ms = min(r["ms_per_step"] for r in table)
+    out = dict(shape=best_shape, ms_per_step=best_ms, table=table, final=final, steps=steps,
                warmup=warmup)
     if cache is not None:
         cache = Path(cache)
