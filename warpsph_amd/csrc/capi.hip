@@ -1290,7 +1290,7 @@ static int device_loop(wsph_solver *s, const Bufs &b, int src, int queued, int m
     const int cnt = C < max_it - queued ? C : max_it - queued;
     hipEvent_t ev3[3] = {nullptr, nullptr, nullptr};
     if (queued == 0) kernel_events(s, ev3);
-    launch_jacobi_chunk(b, s->C, P, src, cnt, P.block, s->stream, ev3);
+    launch_jacobi_chunk(b, s->C, P, src, queued, cnt, P.block, s->stream, ev3);
     const int before = queued;  // iterations of the chunks before this one
     queued += cnt;
     if (queued >= max_it) break;
@@ -1300,6 +1300,9 @@ static int device_loop(wsph_solver *s, const Bufs &b, int src, int queued, int m
       if (rc == 1) break;
     }
   }
+  // every iteration queued: the last one's stop test (the earlier ones ran in
+  // the next iteration's J1)
+  if (queued >= max_it) launch_jacobi_last_test(b, P, src, max_it - 1, s->stream);
   launch_batch_finalize(b, s->stream);
   s->final_dp = -1;
   return WSPH_OK;
@@ -1476,7 +1479,7 @@ static int enqueue_front(wsph_solver *s, int c) {
   s->graph_src = src;
   const int max_it = P.force_iters > 0 ? P.force_iters : P.max_iters;
   const int K = P.graph_iters < max_it ? P.graph_iters : max_it;
-  launch_jacobi_chunk(b, s->C, P, src, K, P.block, s->stream);
+  launch_jacobi_chunk(b, s->C, P, src, 0, K, P.block, s->stream);
   HIP_TRY(hipGetLastError());
   return WSPH_OK;
 }

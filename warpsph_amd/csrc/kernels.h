@@ -112,11 +112,24 @@ void launch_learned_start(const Bufs &b, const SphConst &c, const Weights *net, 
 // evaluation of A p0 (counted in DevState::ap_evals).
 void launch_j1(const Bufs &b, const SphConst &c, int src, int block, hipStream_t s);
 void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, hipStream_t s);
-// `iters` Jacobi iterations with device-side stop test (no-ops once done);
-// ev3 (optional): events recorded before J1, between J1 and J2, after J2 of
-// the chunk's first iteration
+// Stop rule of the device-driven loop (wsph_params' iteration limits and eta).
+struct LoopCtl {
+  int min_it, max_it, force_it;
+  double eta;
+};
+inline LoopCtl loop_ctl(const wsph_params &P) {
+  return LoopCtl{P.min_iters, P.max_iters, P.force_iters, P.eta};
+}
+// Jacobi iterations k0 .. k0 + iters - 1 with the device-side stop test
+// (no-ops once done): one scene tests iteration k - 1 in J1 of iteration k,
+// batched scenes run a stop kernel after each J2. ev3 (optional): events
+// recorded before J1, between J1 and J2, after J2 of the chunk's first
+// iteration.
 void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P, int src0,
-                         int iters, int block, hipStream_t s, hipEvent_t *ev3 = nullptr);
+                         int k0, int iters, int block, hipStream_t s, hipEvent_t *ev3 = nullptr);
+// One scene: stop test of iteration k, the last one the host queued (its
+// test has no next J1 to run it); nothing once the loop has ended.
+void launch_jacobi_last_test(const Bufs &b, const wsph_params &P, int src0, int k, hipStream_t s);
 // Step-end bookkeeping done by the statistics kernel after the kick/drift:
 // clock and CFL dt update (IISPH.py:1367-1371) and the cumulative counters.
 struct StepEnd {

@@ -792,15 +792,12 @@ __device__ __forceinline__ void block_residual_store(const Bufs &b, unsigned lon
   }
 }
 
-// Sum of the block partials of iteration `slot` into DevState; with
-// `check` set, also the stop test of the device-driven loop.
-__global__ void k_residual_reduce(Bufs b, int nparts, int slot, int check, int src0, int min_it,
-                                  int max_it, int force_it, double eta) {
+// Sum of the block partials of iteration `slot` into DevState (the host-driven
+// loop, which then reads the sums and runs the stop test itself).
+__global__ void k_residual_reduce(Bufs b, int nparts, int slot) {
   __shared__ unsigned long long s_sum[16];
   __shared__ unsigned s_cnt[16];
   DevState *st = b.st;
-  if (check && st->done) return;
-  const int k = check ? st->iter : slot;
   unsigned long long S = 0;
   unsigned Cn = 0;
   for (int q = threadIdx.x; q < nparts; q += blockDim.x) {
@@ -821,24 +818,8 @@ __global__ void k_residual_reduce(Bufs b, int nparts, int slot, int check, int s
       S += s_sum[w];
       Cn += s_cnt[w];
     }
-    st->res_sum[k] = S;
-    st->res_cnt[k] = Cn;
-    if (check) {
-      double e;
-      const int cont = jacobi_continue(k, S, Cn, min_it, max_it, force_it, eta, &e);
-      if (cont) {
-        st->iter = k + 1;
-      } else {
-        st->done = 1;
-        st->iterations = k + 1;
-        st->ap_evals += k + 1;
-        st->residual = e;
-        st->converged = e <= eta;
-        st->final_src = src0 ^ (k & 1);
-        st->done_tag = st->step_tag;
-    if (b.done_host) *(volatile int *)b.done_host = st->step_tag;
-      }
-    }
+    st->res_sum[slot] = S;
+    st->res_cnt[slot] = Cn;
   }
 }
 
@@ -903,7 +884,7 @@ __device__ __forceinline__ int batch_decide(const Bufs &b, int k, int src_k, int
   return __syncthreads_or(running);
 }
 
-// One block: the batched counterpart of k_residual_reduce's stop test.
+// One block: the per-scene stop test of a batched device-driven iteration.
 __global__ void k_batch_stop(Bufs b, int src0, int min_it, int max_it, int force_it, double eta) {
   __shared__ unsigned char s_done[kMaxScenes];
   __shared__ double s_emax[16];
@@ -1013,30 +994,92 @@ __device__ __forceinline__ void report_progress(const Bufs &b, const DevState *s
         ((unsigned long long)(unsigned)st->step_tag << 32) | (unsigned)iters;
 }
 
-// Each iteration is J1, J2 and a one-thread check; the kernels read the
-// iteration counter and the done flag from DevState and return at once when
-// the loop has finished, so the host can enqueue iterations ahead without
-// knowing when the loop stops.
+// Loop end of the device-driven Jacobi loop after iteration k (IISPH.py:1200-1207):
+// the iterate of iteration k (buffer src0 ^ (k & 1)) is final; its update,
+// already written to the other buffer, is discarded.
+__device__ __forceinline__ void record_loop_end(const Bufs &b, DevState *st, int k, double e,
+                                                int src0, double eta) {
+  st->done = 1;
+  st->iterations = k + 1;
+  st->ap_evals += k + 1;
+  st->residual = e;
+  st->converged = e <= eta;
+  st->final_src = src0 ^ (k & 1);
+  st->done_tag = st->step_tag;
+  if (b.done_host) *(volatile int *)b.done_host = st->step_tag;
+}
+
+// Residual sum and count of the J2 block partials (res_part / cnt_part,
+// nparts of them), summed by every thread of the calling block (integer sums:
+// exact, any order).
+__device__ __forceinline__ void block_sum_partials(const Bufs &b, int nparts,
+                                                   unsigned long long &S, unsigned &Cn) {
+  __shared__ unsigned long long s_sum[16];
+  __shared__ unsigned s_cnt[16];
+  S = 0;
+  Cn = 0;
+  for (int q = threadIdx.x; q < nparts; q += blockDim.x) {
+    S += b.res_part[q];
+    Cn += b.cnt_part[q];
+  }
+  S = wave_sum_u64(S);
+  Cn = wave_sum_u32(Cn);
+  if ((threadIdx.x & 63) == 0) {
+    s_sum[threadIdx.x >> 6] = S;
+    s_cnt[threadIdx.x >> 6] = Cn;
+  }
+  __syncthreads();
+  S = 0;
+  Cn = 0;
+  for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
+    S += s_sum[w];
+    Cn += s_cnt[w];
+  }
+}
+
+// Each iteration k is J1 and J2. One scene: J2's blocks store their residual
+// partials, and J1 of iteration k + 1 runs the stop test of iteration k: every
+// block sums the same partials and takes the same decision, and block 0
+// records the loop's end. The loop's last iteration is tested by
+// k_jacobi_last_test. Batched scenes keep a one-block stop kernel per
+// iteration (k_batch_stop), which advances DevState::iter. Kernels return at
+// once once the loop has finished, so the host can enqueue iterations ahead
+// without knowing when the loop stops.
 template <bool kLds>
-__global__ void k_j1_dev(Bufs b, SphConst c, int src0) {
-  if (__hip_atomic_load(&b.st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+__global__ void k_j1_dev(Bufs b, SphConst c, int src0, int k, LoopCtl L) {
+  DevState *st = b.st;
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  if (b.scenes) {
+    k = st->iter;
+  } else if (k > 0) {
+    unsigned long long S;
+    unsigned Cn;
+    block_sum_partials(b, (int)gridDim.x, S, Cn);
+    double e;
+    if (!jacobi_continue(k - 1, S, Cn, L.min_it, L.max_it, L.force_it, L.eta, &e)) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->res_sum[k - 1] = S;
+        st->res_cnt[k - 1] = Cn;
+        record_loop_end(b, st, k - 1, e, src0, L.eta);
+      }
+      return;
+    }
+  }
   const Tabs T = stage_tabs<kLds>(b, c);
-  const int k = b.st->iter;
-  if (k > 0 && blockIdx.x == 0 && threadIdx.x == 0) report_progress(b, b.st, k);
+  if (k > 0 && blockIdx.x == 0 && threadIdx.x == 0) report_progress(b, st, k);
   const int i = particle_index();
-  const float dt = (float)b.st->dt;
+  const float dt = (float)st->dt;
   const float t3 = c.mass * (dt * dt);
   int scene;
   if (scene_active(b, i, scene)) j1_particle(b, c, T, i, src0 ^ (k & 1), t3);
 }
 
-// J2 of a device-driven iteration; k_residual_reduce then runs the stop test
-// (IISPH.py:1200-1207).
+// J2 of a device-driven iteration k (batched scenes: DevState::iter).
 template <bool kLds>
-__global__ void k_j2_dev(Bufs b, SphConst c, int src0) {
+__global__ void k_j2_dev(Bufs b, SphConst c, int src0, int k) {
   if (__hip_atomic_load(&b.st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const Tabs T = stage_tabs<kLds>(b, c);
-  const int k = b.st->iter;
+  if (b.scenes) k = b.st->iter;
   const int i = particle_index();
   unsigned long long rs = 0;
   unsigned rc = 0;
@@ -1047,6 +1090,23 @@ __global__ void k_j2_dev(Bufs b, SphConst c, int src0) {
     scene_residual_add(b, k, act ? scene : -1, rs, rc);
   else
     block_residual_store(b, rs, rc);
+}
+
+// Stop test of the last queued iteration k of a one-scene device loop (the
+// test of every earlier iteration runs in the next J1).
+__global__ void k_jacobi_last_test(Bufs b, int nparts, int src0, int k, LoopCtl L) {
+  DevState *st = b.st;
+  if (st->done) return;
+  unsigned long long S;
+  unsigned Cn;
+  block_sum_partials(b, nparts, S, Cn);
+  double e;
+  if (threadIdx.x == 0 &&
+      !jacobi_continue(k, S, Cn, L.min_it, L.max_it, L.force_it, L.eta, &e)) {
+    st->res_sum[k] = S;
+    st->res_cnt[k] = Cn;
+    record_loop_end(b, st, k, e, src0, L.eta);
+  }
 }
 
 // ---- kick (IISPH.py:674-688) and drift (IISPH.py:691-699) ----------------
@@ -1551,25 +1611,28 @@ void launch_j2(const Bufs &b, const SphConst &c, int src, int slot, int block, h
   if (b.n == 0) return;
   DISPATCH_LDS(k_j2, nblocks(b.n, block), block, j2_lds_bytes(b, c, block), s, b, c, src, slot);
   if (slot >= 0)
-    k_residual_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block), slot, 0, 0, 0, 0, 0, 0.0);
+    k_residual_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block), slot);
 }
 
 void launch_jacobi_chunk(const Bufs &b, const SphConst &c, const wsph_params &P, int src0,
-                         int iters, int block, hipStream_t s, hipEvent_t *ev3) {
+                         int k0, int iters, int block, hipStream_t s, hipEvent_t *ev3) {
   if (b.n == 0) return;
-  for (int k = 0; k < iters; ++k) {
-    const bool ev = ev3 && ev3[0] && k == 0;
+  const LoopCtl L = loop_ctl(P);
+  for (int k = k0; k < k0 + iters; ++k) {
+    const bool ev = ev3 && ev3[0] && k == k0;
     if (ev) (void)hipEventRecord(ev3[0], s);
-    DISPATCH_LDS(k_j1_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0);
+    DISPATCH_LDS(k_j1_dev, nblocks(b.n, block), block, tab_bytes(b, c), s, b, c, src0, k, L);
     if (ev && ev3[1]) (void)hipEventRecord(ev3[1], s);
-    DISPATCH_LDS(k_j2_dev, nblocks(b.n, block), block, j2_lds_bytes(b, c, block), s, b, c, src0);
+    DISPATCH_LDS(k_j2_dev, nblocks(b.n, block), block, j2_lds_bytes(b, c, block), s, b, c, src0, k);
     if (ev) (void)hipEventRecord(ev3[2], s);
     if (b.scenes)
       k_batch_stop<<<1, 1024, 0, s>>>(b, src0, P.min_iters, P.max_iters, P.force_iters, P.eta);
-    else
-      k_residual_reduce<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, block), 0, 1, src0, P.min_iters,
-                                           P.max_iters, P.force_iters, P.eta);
   }
+}
+
+void launch_jacobi_last_test(const Bufs &b, const wsph_params &P, int src0, int k, hipStream_t s) {
+  if (b.n == 0 || b.scenes || k < 0) return;
+  k_jacobi_last_test<<<1, 1024, 0, s>>>(b, (int)nblocks(b.n, P.block), src0, k, loop_ctl(P));
 }
 
 void launch_batch_finalize(const Bufs &b, hipStream_t s) {
