@@ -184,6 +184,7 @@ struct wsph_solver {
   double *ke_part = nullptr, *y_part = nullptr;
   StepRecord *history = nullptr;
   unsigned long long *res_part = nullptr;
+  int *list_max = nullptr;  // kListMaxSlots slots of the list pass's largest counts
   unsigned *cnt_part = nullptr;
   bool have_p_ext = false;
   int *slot = nullptr, *rank = nullptr, *tmp = nullptr, *perm = nullptr;
@@ -316,6 +317,7 @@ static Bufs make_bufs(wsph_solver *s) {
   b.y_part = s->y_part;
   b.history = s->history;
   b.res_part = s->res_part;
+  b.list_max = s->list_max;
   b.cnt_part = s->cnt_part;
   b.fstart = s->fg.start;
   b.fcount = s->fg.count;
@@ -531,6 +533,8 @@ int wsph_create(const wsph_params *params, wsph_solver **out) {
   CREATE_TRY(hipDeviceGetAttribute(&s->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
   CREATE_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   CREATE_TRY(dalloc(&s->st, 1));
+  CREATE_TRY(dalloc(&s->list_max, (size_t)kListMaxSlots * kListMaxStride));
+  CREATE_TRY(hipMemsetAsync(s->list_max, 0, sizeof(int) * kListMaxSlots * kListMaxStride, s->stream));
   CREATE_TRY(hipHostMalloc((void **)&s->st_host, sizeof(DevState), hipHostMallocDefault));
   CREATE_TRY(dalloc(&s->totals, 8));
   CREATE_TRY(dalloc(&s->history, kHistory));
@@ -653,6 +657,7 @@ int wsph_destroy(wsph_solver *s) {
   dfree(s->wt);
   dfree(s->gt);
   dfree(s->st);
+  dfree(s->list_max);
   dfree(s->totals);
   dfree(s->history);
   if (s->st_host) (void)hipHostFree(s->st_host);

@@ -54,6 +54,12 @@ struct SceneGrid {
   int fbase, bbase;
 };
 
+// Slots of the list pass's largest neighbour counts (one 128-byte line
+// each): every wave of the pass records its maxima, and device-scope atomics
+// on one word serialise at the memory side, so the waves spread them.
+constexpr int kListMaxSlots = 256;
+constexpr int kListMaxStride = 32;  // ints per slot
+
 // status bits accumulated on the device during a step
 enum : int {
   ST_FF_OVERFLOW = 1,

@@ -54,6 +54,10 @@ struct Bufs {
   unsigned *ymax_part;  // per-block max height (ordered bits)
   StepRecord *history;  // kHistory entries
   unsigned long long *res_part;  // per-block residual sums of J2
+  // largest neighbour counts of the list pass, spread over kListMaxSlots
+  // cache lines (fluid at slot * 32, boundary at slot * 32 + 1); folded into
+  // DevState::max_ff / max_fs at the step's end
+  int *list_max;
   unsigned *cnt_part;            // per-block residual counts of J2
   const float *pstart;  // caller-supplied start pressure, sorted order (or null)
   // pressure predictor (null / 0 when unused)

@@ -394,6 +394,17 @@ __device__ __forceinline__ bool near_boundary(const Bufs &b, const SphConst &c, 
   return b.bnear[(g.start - b.bstart) + sl] != 0u;
 }
 
+// Largest neighbour counts of one wave of the list pass (and the overflow
+// flags, rare): into slot `wave` mod kListMaxSlots.
+__device__ __forceinline__ void record_list_max(const Bufs &b, int wave, int mf, int ms) {
+  if ((threadIdx.x & 63) != 0) return;
+  int *slot = b.list_max + (wave & (kListMaxSlots - 1)) * kListMaxStride;
+  atomicMax(slot, mf);
+  if (ms > 0) atomicMax(slot + 1, ms);
+  if (mf > b.cap_ff) atomicOr(&b.st->status, (int)ST_FF_OVERFLOW);
+  if (ms > b.cap_fs) atomicOr(&b.st->status, (int)ST_FS_OVERFLOW);
+}
+
 __global__ void k_lists(Bufs b, SphConst c) {
   int *ring = reinterpret_cast<int *>(g_smem) + (threadIdx.x >> 6) * (kListRing * 64);
   const int lane = (int)(threadIdx.x & 63);
@@ -430,13 +441,7 @@ __global__ void k_lists(Bufs b, SphConst c) {
     b.cnt_ff[i] = nf < b.cap_ff ? nf : b.cap_ff;
     b.cnt_fs[i] = ns < b.cap_fs ? ns : b.cap_fs;
   }
-  const int mf = wave_max_i32(nf), ms = wave_max_i32(ns);
-  if (lane == 0) {
-    atomicMax(&b.st->max_ff, mf);
-    atomicMax(&b.st->max_fs, ms);
-    if (mf > b.cap_ff) atomicOr(&b.st->status, (int)ST_FF_OVERFLOW);
-    if (ms > b.cap_fs) atomicOr(&b.st->status, (int)ST_FS_OVERFLOW);
-  }
+  record_list_max(b, i / kListLanes, wave_max_i32(nf), wave_max_i32(ns));
 }
 
 // Marks of near_boundary: every slot within 2 cells (each axis) of a boundary
@@ -1226,9 +1231,17 @@ __global__ void k_integrate(Bufs b, SphConst c, int src) {
 // history ring.
 __global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode, StepEnd end) {
   __shared__ unsigned s[16], sy[16];
+  __shared__ int sf[16], sb[16];
   __shared__ double sk[16], ss[16], sn[16], sc[16];
   unsigned m = 0, my = 0;
   double ke = 0.0, ys = 0.0, en = 0.0, ec = 0.0;
+  int lf = 0, lb = 0;  // largest neighbour counts of the step's list pass
+  for (int k = threadIdx.x; k < kListMaxSlots; k += blockDim.x) {
+    lf = max(lf, b.list_max[k * kListMaxStride]);
+    lb = max(lb, b.list_max[k * kListMaxStride + 1]);
+  }
+  lf = wave_max_i32(lf);
+  lb = wave_max_i32(lb);
   // fixed assignment of partials to threads and a fixed combining tree:
   // the same sum on every run
   for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
@@ -1252,6 +1265,8 @@ __global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode, StepEnd end) {
   if ((threadIdx.x & 63) == 0) {
     s[threadIdx.x >> 6] = m;
     sy[threadIdx.x >> 6] = my;
+    sf[threadIdx.x >> 6] = lf;
+    sb[threadIdx.x >> 6] = lb;
     sk[threadIdx.x >> 6] = ke;
     ss[threadIdx.x >> 6] = ys;
     sn[threadIdx.x >> 6] = en;
@@ -1268,6 +1283,8 @@ __global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode, StepEnd end) {
     for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
       m = max(m, s[w]);
       my = max(my, sy[w]);
+      lf = max(lf, sf[w]);
+      lb = max(lb, sb[w]);
       ke += sk[w];
       ys += ss[w];
       en += sn[w];
@@ -1275,6 +1292,8 @@ __global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode, StepEnd end) {
     }
     DevState *st = b.st;
     st->vmax_bits = m;
+    st->max_ff = max(st->max_ff, lf);
+    st->max_fs = max(st->max_fs, lb);
     StepRecord &r = b.history[st->step_count % kHistory];
     r.sim_time = st->sim_time + st->dt;
     r.residual = st->residual;
@@ -1319,11 +1338,15 @@ __global__ void k_vmax_reduce(Bufs b, int nparts, int start_mode, StepEnd end) {
   }
 }
 
-__global__ void k_reset_state(DevState *st) {
+__global__ void k_reset_state(DevState *st, int *list_max) {
   const int t = threadIdx.x;
   for (int k = t; k < kMaxIters; k += blockDim.x) {
     st->res_sum[k] = 0ull;
     st->res_cnt[k] = 0u;
+  }
+  for (int k = t; k < kListMaxSlots; k += blockDim.x) {
+    list_max[k * kListMaxStride] = 0;
+    list_max[k * kListMaxStride + 1] = 0;
   }
   if (t == 0) {
     st->vmax_bits = 0u;
@@ -1647,7 +1670,7 @@ void launch_integrate(const Bufs &b, const SphConst &c, int src, int start_mode,
 }
 
 void launch_reset_state(const Bufs &b, hipStream_t s) {
-  k_reset_state<<<1, 128, 0, s>>>(b.st);
+  k_reset_state<<<1, 128, 0, s>>>(b.st, b.list_max);
   if (b.scenes) k_batch_reset<<<64, 256, 0, s>>>(b, kMaxIters);
 }
 
