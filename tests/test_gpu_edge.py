@@ -1,7 +1,8 @@
 """Edge cases of the HIP path: no fluid, isolated particles, no boundary,
 the neighbour-list writer past its ring and at exact capacities,
 neighbour-list capacity overflow (an error, where the reference overruns its
-n*60 slots), a batch containing an empty scene."""
+n*60 slots), the reported largest neighbour counts, a batch containing an
+empty scene."""
 import sys
 
 import numpy as np
@@ -145,6 +146,28 @@ def test_neighbour_capacity_overflow_is_an_error():
     s2 = _solver(cfg2, scene.build(cfg2))
     s2.step()
     assert s2.step_stats()["max_ff_count"] <= cfg2.max_ff
+
+
+def test_reported_count_maxima_are_each_steps_largest_counts():
+    """The list pass records each wave's largest counts in one of 256 slots
+    (cleared at every step's start, folded at its end): the step statistics
+    equal the largest per-particle counts of that step's lists, also in steps
+    where the maximum falls (a slot kept from an earlier step would show)."""
+    _announce("edge: reported neighbour-count maxima")
+    cfg = config.dambreak_8k()
+    s = _solver(cfg, scene.build(cfg))
+    seen_fall = False
+    prev = None
+    for _ in range(40):
+        s.step()
+        st = s.step_stats()
+        ff = int(s.field(N.F_FF_COUNT).max().item())
+        fs = int(s.field(N.F_FS_COUNT).max().item())
+        assert (st["max_ff_count"], st["max_fs_count"]) == (ff, fs)
+        if prev is not None and ff < prev:
+            seen_fall = True
+        prev = ff
+    assert seen_fall
 
 
 def test_particle_count_limit_is_an_error():
