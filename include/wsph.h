@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define WSPH_ABI_VERSION 1
+#define WSPH_ABI_VERSION 2
 
 /* The mean density error (update_rho_error, IISPH.py:542-568) is summed in
  * 64-bit fixed point so that it is independent of summation order: each
@@ -118,7 +118,7 @@ typedef struct wsph_params {
   int max_fs;       /* fluid-boundary neighbour capacity per particle */
   int jacobi_mode;  /* WSPH_JACOBI_* */
   int block;        /* threads per block of the neighbour passes (64..1024) */
-  int coop_blocks_per_cu; /* persistent kernel blocks per CU (0 = default 2) */
+  int coop_blocks_per_cu; /* persistent kernel blocks per CU (0 = as many as fit) */
   int chunk_iters;  /* WSPH_JACOBI_DEVICE: iterations queued per chunk (0 = 2) */
   int graph_iters;  /* WSPH_JACOBI_DEVICE, > 0: wsph_step replays a captured hipGraph of
                        the step's grid build, advection passes, start value and first
@@ -177,12 +177,18 @@ float wsph_radius_sq_threshold(float h);
  * block counted as taking ceil(waves / 4) slots on every SIMD: exact for
  * blocks of 4 or more waves, a lower bound for smaller ones), the VGPRs
  * (512 per lane and SIMD, allocated in units of 8, vgprs per lane of the
- * kernel), the LDS (lds_bytes per block, static plus dynamic, allocated in
- * units of 512 bytes, against cu_lds_bytes) and 16 workgroups per CU. Host
- * arithmetic only; WSPH_E_ARG for figures out of range. The persistent
- * driver reads the kernel's figures with hipFuncGetAttributes and launches
- * one block per CU fewer than this (at least one). */
-int wsph_blocks_per_cu(int block_threads, int vgprs, int lds_bytes, int cu_lds_bytes,
+ * kernel), the SGPRs, the LDS (lds_bytes per block, static plus dynamic,
+ * allocated in units of 512 bytes, against cu_lds_bytes) and 16 workgroups
+ * per CU. SGPRs: a wave holds its sgprs (the kernel's total, VCC included)
+ * rounded up to 16, plus 16 more, out of 800 per SIMD; measured on MI355X,
+ * kernels of 90 and 106 SGPRs fit 7 and 6 waves per SIMD where the runtime's
+ * occupancy query reports 8 and 7. The runtime does not report a kernel's
+ * SGPRs: sgprs <= 0 takes the most a kernel can hold (112), 6 waves per
+ * SIMD. Host arithmetic only; WSPH_E_ARG for figures out of range. The
+ * persistent driver reads the kernel's VGPRs and LDS with
+ * hipFuncGetAttributes and launches this many blocks per CU (or fewer when
+ * wsph_params.coop_blocks_per_cu asks for fewer). */
+int wsph_blocks_per_cu(int block_threads, int vgprs, int sgprs, int lds_bytes, int cu_lds_bytes,
                        int waves_per_simd);
 
 /* Self-test of the kernels' rounding helpers on the current device (no

@@ -32,7 +32,7 @@ def test_library_exports_every_symbol():
 
 def test_abi_version_and_error_string():
     lib = N.load()
-    assert lib.wsph_abi_version() == 1
+    assert lib.wsph_abi_version() == 2
     assert isinstance(lib.wsph_last_error(), bytes)
 
 
@@ -65,30 +65,44 @@ def test_radius_threshold_matches_sqrt_predicate(h):
 
 def test_blocks_per_cu_from_kernel_resources():
     """The co-residency figure of wsph_blocks_per_cu (host arithmetic, no GPU),
-    on the persistent Jacobi kernel's own resource figures (-Rpass-analysis
-    =kernel-resource-usage of the gfx950 build) and on limits taken one at a
-    time (MI355X: 160 KiB LDS per CU, 8 waves per SIMD)."""
+    on the persistent Jacobi kernel's own resource figures (the gfx950
+    build's kernel-resource-usage remarks) and on limits taken one at a time
+    (MI355X: 160 KiB LDS per CU, 8 waves per SIMD)."""
     lib = N.load()
     lds_cu, wps = 160 * 1024, 8
-    # an earlier build of the kernel: 62 VGPRs, 200 B static + 8,000 B of
-    # tables, 256-thread blocks: the wave slots bound it at 8 per CU, where a
-    # launch once found only 7 resident, so the driver launches at most 7
-    assert lib.wsph_blocks_per_cu(256, 62, 200 + 8000, lds_cu, wps) == 8
-    # the current kernel: 114 VGPRs (4 waves per SIMD), 8,648 B static + 8,000 B
-    # of tables; 512-thread blocks put 2 waves on each SIMD
-    assert lib.wsph_blocks_per_cu(512, 114, 8648 + 8000, lds_cu, wps) == 2
-    assert lib.wsph_blocks_per_cu(256, 114, 8648 + 8000, lds_cu, wps) == 4
+    f = lib.wsph_blocks_per_cu
+    # an earlier build of the kernel: 62 VGPRs, 90 SGPRs (84 + VCC and the
+    # like), 200 B static + 8,000 B of tables, 256-thread blocks: its SGPRs
+    # hold it to 7 per CU, the residency a launch at the occupancy query's 8
+    # found (1,792 of 2,048 blocks); with no SGPR figure, the bound for the
+    # largest SGPR count, 6
+    assert f(256, 62, 90, 200 + 8000, lds_cu, wps) == 7
+    assert f(256, 62, 0, 200 + 8000, lds_cu, wps) == 6
+    # the same VGPRs with few SGPRs: the wave slots, 8
+    assert f(256, 62, 21, 200 + 8000, lds_cu, wps) == 8
+    # the current kernel: 104-111 VGPRs (4 waves per SIMD), 106 SGPRs (6),
+    # 8,648 B static + 8,000 B of tables; 512-thread blocks put 2 waves on each SIMD
+    for sg in (106, 0):
+        assert f(512, 114, sg, 8648 + 8000, lds_cu, wps) == 2
+        assert f(256, 114, sg, 8648 + 8000, lds_cu, wps) == 4
+    # SGPRs: rounded up to 16, plus 16, out of 800 per SIMD -- measured 7 and 6
+    # waves per SIMD at 90 and 106; 80 -> 96 -> 8; 112 (the most) -> 6
+    assert f(64, 16, 90, 0, lds_cu, 32) == 7
+    assert f(64, 16, 106, 0, lds_cu, 32) == 6
+    assert f(64, 16, 80, 0, lds_cu, 32) == 8
+    assert f(64, 16, 112, 0, lds_cu, 32) == 6
+    assert f(64, 16, 113, 0, lds_cu, 32) == N.WSPH_E_ARG
     # LDS: 60,000 B rounds up to 60,416 B, two fit in 160 KiB
-    assert lib.wsph_blocks_per_cu(64, 32, 60000, lds_cu, wps) == 2
+    assert f(64, 32, 21, 60000, lds_cu, wps) == 2
     # VGPRs allocated in units of 8: 129 -> 136 -> 3 waves per SIMD
-    assert lib.wsph_blocks_per_cu(256, 129, 0, lds_cu, wps) == 3
+    assert f(256, 129, 21, 0, lds_cu, wps) == 3
     # 16 workgroups per CU at most; one-wave blocks counted on every SIMD (a
     # lower bound: they could pack 4 times as densely)
-    assert lib.wsph_blocks_per_cu(64, 16, 0, lds_cu, wps) == 8
-    assert lib.wsph_blocks_per_cu(64, 16, 0, lds_cu, 32) == 16
+    assert f(64, 16, 21, 0, lds_cu, wps) == 8
+    assert f(64, 16, 21, 0, lds_cu, 32) == 16
     # a block of 1024 threads (16 waves) needs 4 waves on each SIMD
-    assert lib.wsph_blocks_per_cu(1024, 64, 0, lds_cu, wps) == 2
-    assert lib.wsph_blocks_per_cu(0, 64, 0, lds_cu, wps) == N.WSPH_E_ARG
+    assert f(1024, 64, 21, 0, lds_cu, wps) == 2
+    assert f(0, 64, 21, 0, lds_cu, wps) == N.WSPH_E_ARG
 
 
 def test_mesh_noalloc_rejects_oversized_lattice_before_device_work():

@@ -107,7 +107,7 @@ SIGNATURES = [
     ("wsph_last_error", C.c_char_p, []),
     ("wsph_abi_version", _I, []),
     ("wsph_radius_sq_threshold", C.c_float, [C.c_float]),
-    ("wsph_blocks_per_cu", _I, [_I, _I, _I, _I, _I]),
+    ("wsph_blocks_per_cu", _I, [_I, _I, _I, _I, _I, _I]),
     ("wsph_selftest_arith", _I, [C.POINTER(C.c_ulonglong)]),
     ("wsph_create", _I, [C.POINTER(WsphParams), C.POINTER(_P)]),
     ("wsph_destroy", _I, [_P]),

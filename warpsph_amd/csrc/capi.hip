@@ -444,19 +444,23 @@ const char *wsph_last_error(void) { return g_err; }
 float wsph_radius_sq_threshold(float h) { return radius_sq_threshold(h); }
 int wsph_abi_version(void) { return WSPH_ABI_VERSION; }
 
-int wsph_blocks_per_cu(int block_threads, int vgprs, int lds_bytes, int cu_lds_bytes,
+int wsph_blocks_per_cu(int block_threads, int vgprs, int sgprs, int lds_bytes, int cu_lds_bytes,
                        int waves_per_simd) {
-  if (block_threads < 1 || block_threads > 1024 || vgprs < 0 || lds_bytes < 0 ||
+  if (block_threads < 1 || block_threads > 1024 || vgprs < 0 || sgprs > 112 || lds_bytes < 0 ||
       cu_lds_bytes <= 0 || waves_per_simd <= 0)
     return fail(WSPH_E_ARG, "bad resource figures");
   constexpr int kSimds = 4, kVgprsPerLane = 512, kVgprGranule = 8, kLdsGranule = 512,
-                kMaxGroups = 16;
+                kMaxGroups = 16, kSgprsPerSimd = 800, kSgprGranule = 16, kSgprExtra = 16,
+                kSgprMax = 112;
   const int waves = (block_threads + 63) / 64;
   // a block's waves spread over the 4 SIMDs: at most ceil(waves / 4) on each
   const int per_simd = (waves + kSimds - 1) / kSimds;
   int wps = waves_per_simd;
   const int v = (vgprs + kVgprGranule - 1) / kVgprGranule * kVgprGranule;
   if (v > 0) wps = wps < kVgprsPerLane / v ? wps : kVgprsPerLane / v;
+  const int sg = sgprs > 0 ? sgprs : kSgprMax;
+  const int s = (sg + kSgprGranule - 1) / kSgprGranule * kSgprGranule + kSgprExtra;
+  wps = wps < kSgprsPerSimd / s ? wps : kSgprsPerSimd / s;
   int blocks = wps / per_simd;
   const int l = (lds_bytes + kLdsGranule - 1) / kLdsGranule * kLdsGranule;
   if (l > 0) blocks = blocks < cu_lds_bytes / l ? blocks : cu_lds_bytes / l;
@@ -1337,13 +1341,10 @@ int wsph_pressure_solve(wsph_solver *s, wsph_step_stats *stats) {
     const int per = persistent_blocks_per_cu(b, s->C, P.block, &maxb);
     if (maxb <= 0 || per <= 0)
       return fail(WSPH_E_UNSUPPORTED, "persistent kernel does not fit on the device");
-    // Stay one block per CU below the computed figure: a launch at the full
-    // figure (8 blocks of 256 per CU, 2048 blocks) once found only 1792 blocks
-    // resident, a shortfall that the kernel's registers, LDS and the wave and
-    // workgroup slots do not account for.
-    int eff = P.coop_blocks_per_cu > 0 ? P.coop_blocks_per_cu : 2;
-    const int cap = per > 1 ? per - 1 : 1;
-    eff = eff < cap ? eff : cap;
+    // the computed figure, SGPRs included (a launch at the occupancy query's
+    // figure, which leaves them out, once found 1792 of 2048 blocks resident)
+    int eff = P.coop_blocks_per_cu > 0 ? P.coop_blocks_per_cu : per;
+    eff = eff < per ? eff : per;
     const int ncu = maxb / per;
     const int need = (int)nblk(s->n, P.block);
     const int grid = need < eff * ncu ? need : eff * ncu;

@@ -1696,7 +1696,8 @@ int persistent_blocks_per_cu(const Bufs &b, const SphConst &c, int block, int *t
                            : hipFuncGetAttributes(&fa, (const void *)k_jacobi_persistent<false>);
   if (e != hipSuccess) return -1;
   const int lds = (int)(fa.sharedSizeBytes + tab_bytes(b, c));
-  const int per = wsph_blocks_per_cu(block, fa.numRegs, lds, lds_cu, threads_cu / 64 / 4);
+  // the runtime reports no SGPR count: 0 takes the largest a kernel can hold
+  const int per = wsph_blocks_per_cu(block, fa.numRegs, 0, lds, lds_cu, threads_cu / 64 / 4);
   if (total) *total = per * ncu;
   return per;
 }
