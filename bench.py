@@ -482,13 +482,21 @@ def main():
     ap_evals = r["ap_evals"]
     ktm, kernel_source = tm, "the timed run (every 4th step)"
     if int(tm[11]) == 0:
-        # graph replay: the first iteration's J1 and J2 run inside the captured
-        # graph, where no events are recorded; time the same kernels in a short
-        # eager run of the same shape
+        # graph replay or the persistent kernel: the first iteration's J1 and J2
+        # run inside the captured graph or the one cooperative launch, where no
+        # events are recorded; time the same kernels in a short eager run of
+        # the same shape with the device-driven loop
         short = argparse.Namespace(**vars(args))
         short.steps, short.warmup = min(args.steps, 100), args.warmup
-        q = timed_run(cfg.replace(graph_iters=0), net, short, trace=False)
-        ktm, kernel_source = q["tm"], f"an eager run of the same shape ({short.steps} steps)"
+        eager = cfg.replace(graph_iters=0)
+        if eager.jacobi_mode == config.JACOBI_PERSISTENT:
+            eager = eager.replace(jacobi_mode=config.JACOBI_DEVICE, chunk_iters=1,
+                                  coop_blocks_per_cu=0)
+        q = timed_run(eager, net, short, trace=False)
+        ktm = q["tm"]
+        kernel_source = (f"an eager run of the same shape ({short.steps} steps)"
+                         if eager.jacobi_mode == cfg.jacobi_mode else
+                         f"an eager device-driven run of the same block shape ({short.steps} steps)")
         q["solver"].close()
     ksteps = max(1, int(ktm[11]))
     pair_ms = ktm[10] / ksteps
