@@ -587,6 +587,13 @@ def main():
         ref["host_cpu"] = _cpu_model()
         ref["nproc"] = _nproc()
         line["cpu_reference_dims_all_cores"] = ref
+        # the reference's algorithm and hash dims on one thread, as Warp's CPU launches
+        # run (one step, ~30 s at 1M particles)
+        one = cpu_baseline(ccfg.replace(grid="reference"), x_end, v_end, 0.0, threads=1,
+                           boundary=b_end)
+        one["host_cpu"] = _cpu_model()
+        one["nproc"] = _nproc()
+        line["cpu_reference_faithful"] = one
     print(json.dumps(line), flush=True)
 
 
