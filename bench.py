@@ -25,7 +25,10 @@ launches of every timed step's first Jacobi iteration.
 
 The CPU baseline times the float32 CPU oracle (one thread) on a bounded
 sample: the GPU's particle state at the end of the timed run, advanced a
-few steps.
+few steps. Beside it: the same on every host core, with the reference's hash
+dims on every core, and one step with the reference's hash dims on one thread
+(the reference's algorithm as Warp's serial CPU launches run it;
+--no-cpu-faithful skips that one).
 
 ``--preset mesh_sample`` measures the model sampler instead (model.py:14-47):
 the lattice points inside a closed mesh, a 36,864-triangle torus
@@ -73,6 +76,9 @@ def parse():
                     help="plain starts to run after the headline run (comma list, '' for none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-faithful", action="store_true",
+                    help="skip the one-step, one-thread CPU timing with the reference's hash "
+                         "dims (~36 s at 1M particles)")
     ap.add_argument("--layout", type=int, default=-1, help="cell layout: 0 linear, 1 Morton")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0: solver default)")
     ap.add_argument("--graph-iters", type=int, default=-1,
@@ -587,8 +593,9 @@ def main():
         ref["host_cpu"] = _cpu_model()
         ref["nproc"] = _nproc()
         line["cpu_reference_dims_all_cores"] = ref
+    if not args.no_cpu_baseline and not args.no_cpu_faithful:
         # the reference's algorithm and hash dims on one thread, as Warp's CPU launches
-        # run (one step, ~30 s at 1M particles)
+        # run (one step, ~36 s at 1M particles)
         one = cpu_baseline(ccfg.replace(grid="reference"), x_end, v_end, 0.0, threads=1,
                            boundary=b_end)
         one["host_cpu"] = _cpu_model()

@@ -29,7 +29,8 @@ def test_more_than_one_gpu_is_refused():
 def test_bench_line_fields():
     print("[gpu] bench.py short run (contract fields)", flush=True)
     r = subprocess.run([sys.executable, "-u", str(ROOT / "bench.py"), "--steps", "8", "--warmup", "2",
-                        "--compare", "previous", "--autotune", "off", "--cpu-seconds", "2"],
+                        "--compare", "previous", "--autotune", "off", "--cpu-seconds", "2",
+                        "--no-cpu-faithful"],
                        capture_output=True, text=True, timeout=600, cwd=ROOT)
     assert r.returncode == 0, r.stderr[-3000:]
     d = _last_json(r.stdout)
