@@ -93,3 +93,41 @@ def kernel_grad(xi, xj, h):
     if not 1e-10 < n < h:
         return np.zeros(3)
     return kernel_dw(n, h) * r / n
+
+
+# ---- hydrostatic pressure of a settled layer -----------------------------------
+def hydrostatic_config():
+    """A fluid layer filling the floor of a 0.4 x 0.6 x 0.4 m box: 20 x 15 x 20
+    particles at 0.02 m (a 0.30 m column), one wall layer, g = 9.81, dt = 1e-3,
+    XSPH 0.01, previous-step start. It settles without a dam-break collapse."""
+    return config.dambreak_8k().replace(box=(0.4, 0.6, 0.4), fluid_cells=(20, 15, 20),
+                                        warm_start=config.WARM_PREVIOUS)
+
+
+def hydrostatic_fit(cfg, x, p):
+    """Least-squares p = a y + b over the particles at least 2 spacings from
+    the side walls and the floor and 3 below the top; returns (a, -b / a), the
+    slope (hydrostatics: -rho0 g) and the height where the fit crosses p = 0."""
+    x = np.asarray(x, np.float64)
+    p = np.asarray(p, np.float64)
+    d = cfg.spacing
+    bx, _, bz = cfg.box
+    inner = ((x[:, 0] > 2 * d) & (x[:, 0] < bx - 3 * d) & (x[:, 2] > 2 * d)
+             & (x[:, 2] < bz - 3 * d))
+    top = np.percentile(x[:, 1], 99.5)
+    deep = inner & (x[:, 1] > 2 * d) & (x[:, 1] < top - 3 * d)
+    A = np.vstack([x[deep, 1], np.ones(int(deep.sum()))]).T
+    a, b = np.linalg.lstsq(A, p[deep], rcond=None)[0]
+    return a, -b / a
+
+
+def assert_hydrostatic(cfg, slopes, surfaces):
+    """The time-averaged pressure gradient of the settled layer is rho0 g
+    within 5%, and its zero crossing lies within one spacing of the column
+    height (15 layers of 0.02 m filling from y = -0.01, the wall's half
+    spacing: 0.29 m)."""
+    rho_g = cfg.rho0 * cfg.gravity
+    slope = float(np.mean(slopes))
+    surface = float(np.mean(surfaces))
+    assert abs(-slope - rho_g) < 0.05 * rho_g, (slope, rho_g)
+    assert abs(surface - 0.29) < cfg.spacing, surface

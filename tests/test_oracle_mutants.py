@@ -15,6 +15,8 @@ from pathlib import Path
 
 import pytest
 
+import test_hydrostatic as hydro
+import test_neighbor_properties as nprops
 import test_oracle_pins as pins
 
 ORACLE = Path(__file__).resolve().parents[1] / "oracle"
@@ -68,6 +70,34 @@ C_MUTANTS = [
      ["pin_jacobi_fixed_point"]),
 ]
 
+# mistakes of the neighbour search (IISPH.py:52-163), caught by brute force
+NEIGHBOR_MUTANTS = [
+    ("distance test r <= h (distances one ulp short)",
+     "return sqrtf(dx * dx + dy * dy + dz * dz);",
+     "return nextafterf(sqrtf(dx * dx + dy * dy + dz * dz), 0.0f);",
+     ["test_fluid_sets_equal_brute_force", "test_boundary_sets_equal_brute_force"]),
+    ("distance summed as dx^2 + (dy^2 + dz^2)",
+     "return sqrtf(dx * dx + dy * dy + dz * dz);",
+     "return sqrtf(dx * dx + (dy * dy + dz * dz));",
+     ["test_fluid_sets_equal_brute_force", "test_boundary_sets_equal_brute_force"]),
+    ("query range not clamped to dims cells",
+     "q->xe = xe < q->xs + g->dx - 1 ? xe : q->xs + g->dx - 1;", "q->xe = xe;",
+     ["test_fluid_sets_equal_brute_force", "test_boundary_sets_equal_brute_force"]),
+    ("query range clamped one cell short on y",
+     "q->ye = ye < q->ys + g->dy - 1 ? ye : q->ys + g->dy - 1;",
+     "q->ye = ye < q->ys + g->dy - 1 ? ye : q->ys + g->dy - 2;",
+     ["test_fluid_sets_equal_brute_force", "test_boundary_sets_equal_brute_force"]),
+]
+
+# mistakes of the whole step that only physics catches: the settled layer's
+# pressure gradient must balance gravity (tests/test_hydrostatic.py)
+PHYSICS_MUTANTS = [
+    ("kick: sum_j d_ij p_j dropped from the pressure velocity",
+     "(d_ii[3 * (size_t)i + c] * p[i] + s[3 * (size_t)i + c])",
+     "(d_ii[3 * (size_t)i + c] * p[i])",
+     ["test_oracle_settled_layer_is_hydrostatic"]),
+]
+
 PY_MUTANTS = [
     ("features: the particle's own term left out of the means",
      "sw = w0 + w.sum()\n        f[i, 4:] = (w0 * f[i, :4] + w @ f[j, :4]) / sw",
@@ -92,18 +122,17 @@ def _mutate(text: str, old: str, new: str) -> str:
     return text.replace(old, new)
 
 
-def _expect_caught(run_pins, names):
+def _expect_caught(run_pins, names, module=pins):
     caught = []
     for name in names:
         try:
-            run_pins(getattr(pins, name))
+            run_pins(getattr(module, name))
         except AssertionError:
             caught.append(name)
     assert caught, f"no pin among {names} caught the planted mistake"
 
 
-@pytest.mark.parametrize("name,old,new,names", C_MUTANTS, ids=[m[0] for m in C_MUTANTS])
-def test_c_oracle_pins_catch(orc, tmp_path, name, old, new, names):
+def _build_mutant(tmp_path, old, new):
     src = _mutate((ORACLE / "iisph_oracle.c").read_text(), old, new)
     c = tmp_path / "iisph_oracle.c"
     c.write_text(src.replace('#include "iisph_oracle.h"', f'#include "{ORACLE}/iisph_oracle.h"')
@@ -113,8 +142,30 @@ def test_c_oracle_pins_catch(orc, tmp_path, name, old, new, names):
     subprocess.run(["gcc", "-O1", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
                     "-fopenmp", "-shared", "-o", str(so), str(c), str(ORACLE / "mesh_oracle.c"),
                     "-lm"], check=True, capture_output=True)
+    return so
+
+
+@pytest.mark.parametrize("name,old,new,names", C_MUTANTS, ids=[m[0] for m in C_MUTANTS])
+def test_c_oracle_pins_catch(orc, tmp_path, name, old, new, names):
+    so = _build_mutant(tmp_path, old, new)
     with orc.using(so):
         _expect_caught(lambda pin: pin(orc), names)
+
+
+@pytest.mark.parametrize("name,old,new,names", NEIGHBOR_MUTANTS,
+                         ids=[m[0] for m in NEIGHBOR_MUTANTS])
+def test_neighbor_properties_catch(orc, tmp_path, name, old, new, names):
+    so = _build_mutant(tmp_path, old, new)
+    with orc.using(so):
+        _expect_caught(lambda test: test(orc), names, nprops)
+
+
+@pytest.mark.parametrize("name,old,new,names", PHYSICS_MUTANTS,
+                         ids=[m[0] for m in PHYSICS_MUTANTS])
+def test_hydrostatic_catches(orc, tmp_path, name, old, new, names):
+    so = _build_mutant(tmp_path, old, new)
+    with orc.using(so):
+        _expect_caught(lambda test: test(orc), names, hydro)
 
 
 @pytest.mark.parametrize("name,old,new,names", PY_MUTANTS, ids=[m[0] for m in PY_MUTANTS])
