@@ -2,6 +2,7 @@
 (No compute call: this runs without a GPU.)"""
 import ctypes
 import re
+import subprocess
 from pathlib import Path
 
 import pytest
@@ -32,8 +33,24 @@ def test_library_exports_every_symbol():
 
 def test_abi_version_and_error_string():
     lib = N.load()
-    assert lib.wsph_abi_version() == 2
+    header = (Path(N.__file__).resolve().parents[1] / "include" / "wsph.h").read_text()
+    declared = int(re.search(r"#define WSPH_ABI_VERSION (\d+)", header).group(1))
+    assert lib.wsph_abi_version() == declared == N.ABI_VERSION == 2
     assert isinstance(lib.wsph_last_error(), bytes)
+
+
+def test_loader_refuses_another_abi_version(tmp_path):
+    """A stale build (a library of another WSPH_ABI_VERSION, whose entry points
+    may take other arguments) fails at load, not at its first call: a stub
+    exporting every bound symbol and reporting version 1."""
+    src = tmp_path / "stub.c"
+    src.write_text("".join(f"int {name}(void) {{ return 0; }}\n"
+                           for name, _, _ in N.SIGNATURES if name != "wsph_abi_version")
+                   + "int wsph_abi_version(void) { return 1; }\n")
+    so = tmp_path / "libstub.so"
+    subprocess.run(["gcc", "-shared", "-fPIC", str(src), "-o", str(so)], check=True)
+    with pytest.raises(RuntimeError, match="ABI version 1"):
+        N.load(so)
 
 
 def test_create_rejects_bad_params_without_gpu_work():

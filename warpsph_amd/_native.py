@@ -148,6 +148,9 @@ SIGNATURES = [
     ("wsph_sync", _I, [_P]),
 ]
 
+#: WSPH_ABI_VERSION of include/wsph.h, the version SIGNATURES describe
+ABI_VERSION = 2
+
 _lib = None
 
 
@@ -183,6 +186,11 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    got = lib.wsph_abi_version()
+    if got != ABI_VERSION:
+        raise RuntimeError(
+            f"{p} has ABI version {got}, these bindings describe version {ABI_VERSION}: "
+            "rebuild it with `python -m warpsph_amd.build`")
     if path is None:
         _lib = lib
     return lib
