@@ -187,6 +187,14 @@ def dambreak_1m() -> SolverConfig:
         jitter=0.01, seed=0, steps=500)
 
 
+def dambreak_16m() -> SolverConfig:
+    """The 1M dam break's geometry at 0.004 m: 250^3 = 15,625,000 fluid
+    particles in the same 2x1x1 m box with one wall layer, jittered by +-1% of
+    the spacing from seed 0; dt = 4e-4 s fixed (the 1M scene's dt scaled with
+    the spacing); 500 steps. A problem-size point: ~18 GB of device memory."""
+    return dambreak_1m().replace(spacing=0.004, fluid_cells=(250, 250, 250), dt=4e-4)
+
+
 def torus_4m() -> SolverConfig:
     """160^3 = 4,096,000 fluid particles at 0.005 m, block corner at
     (0.2, 0.25, 0.2), falling onto a torus (R = 0.3, r = 0.1, centre
@@ -251,6 +259,7 @@ PRESETS = {
     "reference": reference,
     "dambreak_8k": dambreak_8k,
     "dambreak_1m": dambreak_1m,
+    "dambreak_16m": dambreak_16m,
     "torus_4m": torus_4m,
     "batch_256": batch_256,
     "learned_1m": learned_1m,
