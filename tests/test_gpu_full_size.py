@@ -1,6 +1,7 @@
-"""Bit-exact parity at full size: the 1,000,000-particle dam break (three
-steps) and the 15,625,000-particle one at 0.004 m (one step) against the CPU
-oracle, every per-particle field of the last step compared. The Jacobi
+"""Bit-exact parity at full size against the CPU oracle: the
+1,000,000-particle dam break (three steps), the 15,625,000-particle one at
+0.004 m (one step), the 4,096,000-particle torus-and-plane scene (one step) and
+all 256 scenes of the batched workload (two steps). The Jacobi
 iteration count is forced (4 per step) so that the oracle, OpenMP on the
 host, finishes in seconds; the stop rule is compared at the smaller sizes
 (tests/test_gpu_parity.py). This covers what only full size exercises: list
@@ -46,3 +47,44 @@ def test_full_size_matches_oracle(orc, preset, steps):
     assert_same("v", s.v.cpu().numpy(), o.v)
     assert np.all(np.isfinite(o.x))
     s.close()
+
+
+def test_torus_4m_matches_oracle(orc):
+    """The 4,096,000-particle torus-and-plane scene (no walls, Poisson-disk
+    sampled surfaces, boundary grid apart from the fluid's): one step."""
+    from warpsph_amd.solver import IISPH
+    cfg = config.torus_4m().replace(warm_start=config.WARM_PREVIOUS, force_iters=4)
+    sc = scene.build(cfg)
+    _announce(f"full size torus_4m: {sc.n} fluid, {sc.nb} boundary")
+    s = IISPH(stage_path=None, config=cfg, scene=sc, collect_stats=True)
+    o = orc.OracleSolver(cfg, sc)
+    s.step()
+    info = o.step()
+    for name, t in dict(rho=s.rho, p=s.p, ff_count=s.ff_neighbor_num,
+                        fs_count=s.fs_neighbor_num).items():
+        assert_same(name, t.cpu().numpy(), info[name])
+    assert_same("x", s.x.cpu().numpy(), o.x)
+    assert_same("v", s.v.cpu().numpy(), o.v)
+    s.close()
+
+
+def test_c3_batch_matches_oracle(orc):
+    """All 256 scenes of the batched workload (4,747,246 fluid particles in one
+    batched solver), two steps, each scene against its own oracle run."""
+    from warpsph_amd.batch import BatchIISPH
+    cfgs = [c.replace(warm_start=config.WARM_PREVIOUS, force_iters=4)
+            for c in scene.batch_configs(config.batch_256())]
+    scs = [scene.build(c) for c in cfgs]
+    _announce(f"full size batch: {len(cfgs)} scenes, {sum(sc.n for sc in scs)} fluid")
+    b = BatchIISPH(cfgs, scs)
+    orcs = [orc.OracleSolver(c, sc) for c, sc in zip(cfgs, scs)]
+    for k in range(2):
+        _announce(f"full size batch: step {k + 1}")
+        b.step()
+        xs, vs, ps = (b.split(t.cpu().numpy()) for t in (b.x, b.v, b.p))
+        for q, o in enumerate(orcs):
+            o.step()
+            assert_same(f"step {k} scene {q} x", xs[q], o.x)
+            assert_same(f"step {k} scene {q} v", vs[q], o.v)
+            assert_same(f"step {k} scene {q} p", ps[q], o.p)
+    b.close()
