@@ -1,7 +1,8 @@
 """Bit-exact parity at full size against the CPU oracle: the
 1,000,000-particle dam break (three steps), the 15,625,000-particle one at
-0.004 m (one step), the 4,096,000-particle torus-and-plane scene (one step) and
-all 256 scenes of the batched workload (two steps). The Jacobi
+0.004 m (one step), the 1M scene with a moving paddle (two steps), the
+4,096,000-particle torus-and-plane scene (one step) and all 256 scenes of the
+batched workload (two steps). The Jacobi
 iteration count is forced (4 per step) so that the oracle, OpenMP on the
 host, finishes in seconds; the stop rule is compared at the smaller sizes
 (tests/test_gpu_parity.py). This covers what only full size exercises: list
@@ -24,7 +25,7 @@ def _announce(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("preset,steps", [("dambreak_1m", 3), ("dambreak_16m", 1)])
+@pytest.mark.parametrize("preset,steps", [("dambreak_1m", 3), ("dambreak_16m", 1), ("paddle_1m", 2)])
 def test_full_size_matches_oracle(orc, preset, steps):
     from warpsph_amd.solver import IISPH
     cfg = config.preset(preset).replace(warm_start=config.WARM_PREVIOUS, force_iters=4,
