@@ -1,8 +1,9 @@
 """Bit-exact parity at full size against the CPU oracle: the
 1,000,000-particle dam break (three steps), the 15,625,000-particle one at
 0.004 m (one step), the 1M scene with a moving paddle (two steps), the
-4,096,000-particle torus-and-plane scene (one step) and all 256 scenes of the
-batched workload (two steps). The Jacobi
+4,096,000-particle torus-and-plane scene (one step), all 256 scenes of the
+batched workload (two steps), and the 1M dam break with the learned start run
+by a constant network (three steps). The Jacobi
 iteration count is forced (4 per step) so that the oracle, OpenMP on the
 host, finishes in seconds; the stop rule is compared at the smaller sizes
 (tests/test_gpu_parity.py). This covers what only full size exercises: list
@@ -89,3 +90,44 @@ def test_c3_batch_matches_oracle(orc):
             assert_same(f"step {k} scene {q} v", vs[q], o.v)
             assert_same(f"step {k} scene {q} p", ps[q], o.p)
     b.close()
+
+
+def test_learned_start_constant_network_1m(orc):
+    """The learned start at 1M with a constant network (every weight 0, output
+    bias b3): the matrix cores return the bias exactly, so the start value is
+    max(0, carry + P_ref b3), and the oracle, handed that start (or the carried
+    pressure, as the arbiter chose), reproduces the trajectory bit for bit.
+    This runs the feature gather, the inline MFMA inference, the start's
+    guards and the arbiter at full size."""
+    import learn_oracle as LO
+    import mlp_oracle as MO
+    from warpsph_amd.mlp import PressureNet
+    from warpsph_amd.solver import IISPH
+    b3, steps = 0.05, 3
+    p = np.zeros(MO.N_PARAM, np.float32)
+    p[MO.OFF_B3] = b3
+    net = PressureNet(max_batch=1024, params=p)
+    cfg = config.dambreak_1m().replace(warm_start=config.WARM_LEARNED, force_iters=4)
+    sc = scene.build(cfg)
+    _announce("full size learned start, constant network, 1M")
+    s = IISPH(stage_path=None, config=cfg, scene=sc, collect_stats=True, predictor=net)
+    for _ in range(steps):
+        s.step()
+    hist = s.history()[-steps:]
+    o = orc.OracleSolver(cfg.replace(warm_start=config.WARM_PREVIOUS), sc)
+    carry = np.zeros(sc.n, np.float32)
+    for k in range(steps):
+        _announce(f"full size learned start: oracle step {k + 1}")
+        net_start = LO.start_value(carry, np.float32(b3), np.ones_like(carry), cfg)
+        p0 = net_start if hist[k]["start_source"] == config.WARM_LEARNED else carry
+        info = o.step(p0=p0)
+        # no particle is weakly coupled or inside the a_ii guard band in these steps,
+        # so the start does not depend on a_ii
+        assert np.all(np.abs(info["a_ii"] / LO.a_ref(cfg, cfg.dt0)) >= LO.WEAK_COUPLING)
+        assert hist[k]["iterations"] == info["iterations"] == 4
+        carry = info["p_carry"]
+    assert hist[0]["start_source"] == config.WARM_LEARNED
+    assert_same("x", s.x.cpu().numpy(), o.x)
+    assert_same("v", s.v.cpu().numpy(), o.v)
+    assert_same("p", s.p.cpu().numpy(), info["p"])
+    s.close()
