@@ -48,3 +48,36 @@ def test_boundary_sets_equal_brute_force(orc, fluid, boundary, dims):
         got = idx[i, :cnt[i]].tolist()
         assert len(got) == len(set(got)), i
         assert set(got) == bf[i], i
+
+
+def test_query_bounds_within_two_cells():
+    """The list pass skips a wave's boundary search when none of its lanes' own
+    cells is marked, and the marks reach 2 cells around each boundary
+    particle's cell. That is exact only if a query's float32 range
+    [trunc((p - h) inv), trunc((p + h) inv)] (IISPH.py:52-163's hash-grid
+    query, inv = 1/h in float32) never reaches 3 cells from trunc(p inv). It
+    does reach 2 (next to cell edges), so a 1-cell margin would drop
+    borderline pairs. Checked at every cell edge with |k| < 2^12 and 64 float32
+    neighbours either side, plus 200,000 random positions, for six spacings."""
+    f32 = np.float32
+    rng = np.random.default_rng(0)
+    reach = 0
+    for h in (0.004, 0.008, 0.01, 0.02, 0.04, 0.1):
+        hf = f32(h)
+        inv = f32(1.0) / hf
+        edges = (np.arange(-4096, 4096, dtype=np.float32) * hf).astype(np.float32)
+        ps = [edges]
+        a, b = edges.copy(), edges.copy()
+        for _ in range(64):
+            a = np.nextafter(a, f32(-1e9))
+            b = np.nextafter(b, f32(1e9))
+            ps += [a, b]
+        ps.append(rng.uniform(-4096 * h, 4096 * h, 200_000).astype(np.float32))
+        p = np.concatenate(ps).astype(np.float32)
+        c = np.trunc(p * inv).astype(np.int64)
+        hi = np.trunc((p + hf) * inv).astype(np.int64)
+        lo = np.trunc((p - hf) * inv).astype(np.int64)
+        d = int(max((hi - c).max(), (c - lo).max()))
+        assert d <= 2, (h, d)
+        reach = max(reach, d)
+    assert reach == 2  # the margin of 2 is needed, not only sufficient
